@@ -1,0 +1,153 @@
+// extern "C" device-level ABI (include/xjpcg_dev.h) over the C++ engine core.
+#include "xjpcg_dev.h"
+
+#include <cmath>
+#include <string>
+
+#include "operator.hpp"
+
+struct xjpcg_op {
+    xj::Operator* impl;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F>
+XJPCG_Status_t guarded(F f) {
+    try {
+        f();
+        g_last_error.clear();
+        return XJPCG_STATUS_SUCCESS;
+    } catch (const xj::Error& e) {
+        g_last_error = e.what();
+        return e.status;
+    } catch (const std::exception& e) {
+        g_last_error = std::string("INTERNAL ERROR: ") + e.what();
+        return XJPCG_STATUS_INTERNAL_ERROR;
+    }
+}
+
+inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+xj::Operator& impl_of(xjpcg_op_t* op) {
+    if (op == nullptr || op->impl == nullptr) throw xj::Error(XJPCG_STATUS_NOT_INITIALIZED, "null operator");
+    return *op->impl;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* xjpcg_dev_last_error(void) { return g_last_error.c_str(); }
+
+XJPCG_Status_t xjpcg_op_create(xjpcg_op_t** op, int32_t n, int64_t nnz, const int32_t* d_rowptr,
+                               const int32_t* d_col, const void* d_val, int val_f32, const double* d_dsplit,
+                               const int32_t* h_rowptr, void* stream) {
+    return guarded([&] {
+        if (op == nullptr) throw xj::invalid("null output pointer.");
+        *op = nullptr;
+        auto* impl = new xj::Operator(n, nnz, d_rowptr, d_col, d_val, val_f32 != 0, d_dsplit, h_rowptr,
+                                      as_stream(stream));
+        *op = new xjpcg_op{impl};
+    });
+}
+
+XJPCG_Status_t xjpcg_op_destroy(xjpcg_op_t* op) {
+    if (op == nullptr) return XJPCG_STATUS_NOT_INITIALIZED;
+    delete op->impl;
+    delete op;
+    return XJPCG_STATUS_SUCCESS;
+}
+
+XJPCG_Status_t xjpcg_op_info(const xjpcg_op_t* op, int32_t* n, int64_t* nnz, int32_t* ntiles, int32_t* grid) {
+    return guarded([&] {
+        const xj::Operator& o = impl_of(const_cast<xjpcg_op_t*>(op));
+        if (n) *n = o.csr().n;
+        if (nnz) *nnz = o.csr().nnz;
+        if (ntiles) *ntiles = o.csr().ntiles;
+        if (grid) *grid = o.grid();
+    });
+}
+
+XJPCG_Status_t xjpcg_spmv(xjpcg_op_t* op, const double* d_x, double* d_y, void* stream) {
+    return guarded([&] {
+        if (d_x == nullptr || d_y == nullptr) throw xj::invalid("Vector is nullptr.");
+        impl_of(op).spmv(d_x, d_y, as_stream(stream));
+    });
+}
+
+XJPCG_Status_t xjpcg_residual(xjpcg_op_t* op, const double* d_b, const double* d_x, double* d_r, double* h_rr,
+                              void* stream) {
+    return guarded([&] {
+        if (d_b == nullptr || d_x == nullptr || d_r == nullptr) throw xj::invalid("Vector is nullptr.");
+        double rr = impl_of(op).residual(d_b, d_x, d_r, as_stream(stream));
+        if (h_rr) *h_rr = rr;
+    });
+}
+
+XJPCG_Status_t xjpcg_solve(xjpcg_op_t* op, const double* d_diag, const double* d_b, double* d_x, uint32_t max_iter,
+                           double tol, uint32_t* h_iter, double* h_res, int64_t* h_executed, void* d_records,
+                           void* stream) {
+    return guarded([&] {
+        xj::SolveResult R = impl_of(op).solve(d_diag, d_b, d_x, max_iter, tol, d_records, as_stream(stream));
+        if (h_iter) *h_iter = R.iter;
+        if (h_res) *h_res = R.res;
+        if (h_executed) *h_executed = R.executed;
+    });
+}
+
+XJPCG_Status_t xjpcg_sddmm(xjpcg_op_t* op, const double* d_lam, const double* d_x, double* d_g, void* stream) {
+    return guarded([&] {
+        if (d_lam == nullptr || d_x == nullptr || d_g == nullptr) throw xj::invalid("Vector is nullptr.");
+        impl_of(op).sddmm(d_lam, d_x, d_g, as_stream(stream));
+    });
+}
+
+XJPCG_Status_t xjpcg_solve_mixed(xjpcg_op_t* outer, xjpcg_op_t* inner, const double* d_diag, const double* d_b,
+                                 double* d_x, uint32_t max_outer, double tol, double inner_tol,
+                                 uint32_t inner_max_iter, uint32_t* h_outer, int64_t* h_inner_total,
+                                 double* h_res, void* stream) {
+    uint32_t outer_done = 0;
+    int64_t inner_total = 0;
+    double res = 0.0;
+    XJPCG_Status_t st = guarded([&] {
+        xj::Operator& O = impl_of(outer);
+        xj::Operator& I = impl_of(inner);
+        const int n = O.csr().n;
+        if (I.csr().n != n) throw xj::invalid("inner and outer operators differ in size.");
+        if (d_diag == nullptr || d_b == nullptr || d_x == nullptr) throw xj::invalid("Vector is nullptr.");
+        hipStream_t s = as_stream(stream);
+        double* r = O.aux(0);
+        double* d = O.aux(1);
+        xj::hip_check(hipMemsetAsync(d_x, 0, sizeof(double) * (size_t)n, s), "zero x");
+        // x0 = 0 => r0 = b exactly
+        xj::hip_check(hipMemcpyAsync(r, d_b, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s), "r0 = b");
+        const double bb = O.dot(d_b, d_b, n, s);
+        if (!(bb > 0.0)) {
+            res = 0.0;
+            return;
+        }
+        double rr = bb;
+        res = 1.0;
+        while (true) {
+            res = std::sqrt(rr / bb);
+            if (res < tol || outer_done >= max_outer) break;
+            xj::SolveResult R = I.solve(d_diag, r, d, inner_max_iter, inner_tol, nullptr, s);
+            inner_total += R.executed;
+            xj::hip_check(xj::launch_axpy(n, 1.0, d, d_x, s), "x += d");
+            rr = O.residual(d_b, d_x, r, s);
+            ++outer_done;
+        }
+        if (!(res < tol))
+            throw xj::exec_failed("refinement stopped at relative residual " + std::to_string(res) + " after " +
+                                  std::to_string(outer_done) + " steps.");
+    });
+    if (h_outer) *h_outer = outer_done;
+    if (h_inner_total) *h_inner_total = inner_total;
+    if (h_res) *h_res = res;
+    return st;
+}
+
+}  // extern "C"

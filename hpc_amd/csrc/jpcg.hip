@@ -1,0 +1,309 @@
+// HIP kernels for the xJPCG path on gfx950 and their host launchers.
+//
+//  jpcg_persistent   one cooperative launch per solve: vector set-up, every
+//                    CG iteration and the per-iteration records, separated by
+//                    grid barriers (the reference likewise runs the whole loop
+//                    on the card from one host launch, pcg/sw/src/cgHost.cpp:447-471)
+//  spmv_tiles        standalone y = A x (or y = b - A x), one workgroup per tile
+//  dot_partials / sum_partials   fixed-order two-pass dot product
+//  gather_values     KEEP_NZ_LAYOUT value refresh through the stored permutation
+//  sddmm_tiles       g_k = -lam[row_k] * x[col_k] on the sparsity pattern
+#include "kernels.hpp"
+#include "launch.hpp"
+
+namespace xj {
+
+// --------------------------------------------------------------------------
+// persistent JPCG
+
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void jpcg_persistent(JpcgParams P) {
+    __shared__ double lds[kTileNnz];
+    __shared__ double red[kWaves];
+    __shared__ int sflag;
+    const int G = gridDim.x;
+    const int L = xcd_remap(blockIdx.x, G);
+    const int tid = threadIdx.x;
+    const int gtid = L * kBlock + tid;
+    const int gthreads = G * kBlock;
+    const int n = P.n;
+    const VT* __restrict__ val = static_cast<const VT*>(P.val);
+    double* __restrict__ x = P.x;
+    double* __restrict__ r = P.r;
+    double* __restrict__ p = P.p;
+    double* __restrict__ Ap = P.Ap;
+    double* __restrict__ J = P.J;
+    double* part = P.part;
+    unsigned long long target = 0;
+    // tiles of this workgroup: a contiguous range, so each XCD sweeps one band of rows
+    const int t_begin = (int)((int64_t)P.ntiles * L / G);
+    const int t_end = (int)((int64_t)P.ntiles * (L + 1) / G);
+
+    // set-up (reference cgVector.hpp:101-114): r0 = b, J = 1/diag, z0 = J r0, p0 = z0, x0 = 0
+    double dot = 0.0, rz = 0.0;
+    for (int i = gtid; i < n; i += gthreads) {
+        const double bi = P.b[i];
+        const double Ji = 1.0 / P.diag[i];
+        const double zi = Ji * bi;
+        r[i] = bi;
+        J[i] = Ji;
+        p[i] = zi;
+        x[i] = 0.0;
+        dot += bi * bi;
+        rz += bi * zi;
+    }
+    dot = block_sum(dot, red);
+    rz = block_sum(rz, red);
+    if (tid == 0) {
+        part[3 * G + L] = dot;
+        part[4 * G + L] = rz;
+    }
+    int K = 0;
+    double rr = 0.0;
+    bool ok = grid_barrier(P.bar, target, G, P.abortf, &sflag);
+    if (ok) {
+        dot = grid_total(part + 3 * G, G, red);
+        rz = grid_total(part + 4 * G, G, red);
+    }
+    const double rz0 = rz;
+    const double tols = dot * P.tol * P.tol;
+    const uint32_t vec_size = (uint32_t)((n + 3) / 4 * 4);
+    if (ok && L == 0 && tid == 0 && P.records) {
+        Record rec;
+        rec.id = 0; rec.iter = 0; rec.vec_size = vec_size; rec.max_iter = P.max_rounds;
+        rec.tols = tols; rec.res = dot; rec.rz = rz; rec.alpha = 0.0; rec.beta = 0.0; rec.clock = 0;
+        P.records[0] = rec;
+    }
+    // b == 0: x = 0 is the exact solution; no iteration runs
+    if (ok && dot > 0.0) {
+        uint32_t it = 0;
+        for (;;) {
+            // A: Ap = A p and the p.Ap partial (reference krnl_storeApk.cpp:63-76)
+            double pap = 0.0;
+            for (int t = t_begin; t < t_end; ++t) {
+                double y;
+                int row;
+                tile_spmv<VT>(P.tile_row[t], P.tile_row[t + 1], P.rowptr, P.col, val, P.dsplit, p, lds, red, &y,
+                              &row);
+                if (row >= 0) {
+                    Ap[row] = y;
+                    pap += p[row] * y;
+                }
+            }
+            pap = block_sum(pap, red);
+            if (tid == 0) part[L] = pap;
+            if (!grid_barrier(P.bar, target, G, P.abortf, &sflag)) break;
+            const double alpha = rz / grid_total(part, G, red);
+            // B: x += alpha p; r -= alpha Ap; z = J r; r.r and r.z partials
+            // (reference update_xk.hpp:53-67, update_rk_jacobi.hpp:82-110)
+            double rr_l = 0.0, rz_l = 0.0;
+            for (int i = gtid; i < n; i += gthreads) {
+                const double pi = p[i];
+                x[i] = alpha * pi + x[i];
+                const double ri = -alpha * Ap[i] + r[i];
+                r[i] = ri;
+                const double zi = J[i] * ri;
+                rr_l += ri * ri;
+                rz_l += ri * zi;
+            }
+            rr_l = block_sum(rr_l, red);
+            rz_l = block_sum(rz_l, red);
+            if (tid == 0) {
+                part[G + L] = rr_l;
+                part[2 * G + L] = rz_l;
+            }
+            if (!grid_barrier(P.bar, target, G, P.abortf, &sflag)) break;
+            rr = grid_total(part + G, G, red);
+            const double rzn = grid_total(part + 2 * G, G, red);
+            const double beta = rzn / rz;
+            rz = rzn;
+            ++K;
+            if (L == 0 && tid == 0 && P.records) {
+                Record rec;
+                rec.id = 0; rec.iter = it; rec.vec_size = vec_size; rec.max_iter = P.max_rounds;
+                rec.tols = tols; rec.res = rr; rec.rz = rz; rec.alpha = alpha; rec.beta = beta;
+                rec.clock = wall_clock64();
+                P.records[K] = rec;
+            }
+            // stop rule of control.hpp:83-88 with Task::meetTol / increase (tasks.hpp:43-47)
+            bool more = false;
+            if (!(rr < tols)) {
+                ++it;
+                more = it < P.max_rounds;
+            }
+            if (!more) break;
+            // C: p = beta p + z (reference update_pk.hpp:52-72); x was updated in B, before p changes
+            for (int i = gtid; i < n; i += gthreads) p[i] = beta * p[i] + J[i] * r[i];
+            if (!grid_barrier(P.bar, target, G, P.abortf, &sflag)) break;
+        }
+    }
+    if (L == 0 && tid == 0) {
+        P.result[0] = (double)K;
+        P.result[1] = rr;
+        P.result[2] = dot;
+        P.result[3] = rz0;
+        P.result[4] = (double)__hip_atomic_load(P.abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// --------------------------------------------------------------------------
+// standalone kernels
+
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void spmv_tiles(int ntiles, const int* __restrict__ tile_row,
+                                                     const int* __restrict__ rowptr, const int* __restrict__ col,
+                                                     const VT* __restrict__ val, const double* __restrict__ dsplit,
+                                                     const double* __restrict__ x, double* __restrict__ y,
+                                                     const double* __restrict__ b) {
+    __shared__ double lds[kTileNnz];
+    __shared__ double red[kWaves];
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    if (t >= ntiles) return;
+    double v;
+    int row;
+    tile_spmv<VT>(tile_row[t], tile_row[t + 1], rowptr, col, val, dsplit, x, lds, red, &v, &row);
+    if (row >= 0) y[row] = b ? b[row] - v : v;
+}
+
+// part[blk] = sum over this block's grid-stride slice of a[i]*b[i]
+__global__ __launch_bounds__(kBlock) void dot_partials(int n, const double* __restrict__ a,
+                                                       const double* __restrict__ b, double* __restrict__ part) {
+    __shared__ double red[kWaves];
+    double s = 0.0;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) s += a[i] * b[i];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void sum_partials(int m, const double* __restrict__ part,
+                                                       double* __restrict__ out) {
+    __shared__ double red[kWaves];
+    const double s = grid_total(part, m, red);
+    if (threadIdx.x == 0) *out = s;
+}
+
+template <typename TO, typename TI>
+__global__ __launch_bounds__(kBlock) void gather_values(int64_t nnz, const int64_t* __restrict__ perm,
+                                                        const TI* __restrict__ src, TO* __restrict__ dst) {
+    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * kBlock)
+        dst[k] = (TO)src[perm[k]];
+}
+
+// y = x + a*d  (iterative-refinement update)
+__global__ __launch_bounds__(kBlock) void axpy_kernel(int n, double a, const double* __restrict__ d,
+                                                      double* __restrict__ x) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) x[i] = a * d[i] + x[i];
+}
+
+// g[k] = -lam[row(k)] * x[col[k]] over one tile per workgroup
+__global__ __launch_bounds__(kBlock) void sddmm_tiles(int ntiles, const int* __restrict__ tile_row,
+                                                      const int* __restrict__ rowptr, const int* __restrict__ col,
+                                                      const double* __restrict__ lam,
+                                                      const double* __restrict__ x, double* __restrict__ g) {
+    __shared__ int srow[kTileNnz];
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    if (t >= ntiles) return;
+    const int r0 = tile_row[t], r1 = tile_row[t + 1];
+    const int k0 = rowptr[r0], k1 = rowptr[r1];
+    if (k1 - k0 > kTileNnz) {
+        const double l = -lam[r0];
+        for (int k = k0 + threadIdx.x; k < k1; k += kBlock) g[k] = l * x[col[k]];
+        return;
+    }
+    if ((int)threadIdx.x < r1 - r0) {
+        const int i = r0 + threadIdx.x;
+        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) srow[k - k0] = i;
+    }
+    __syncthreads();
+    for (int k = k0 + threadIdx.x; k < k1; k += kBlock) g[k] = -lam[srow[k - k0]] * x[col[k]];
+}
+
+// --------------------------------------------------------------------------
+// launchers
+
+static int g_coop_grid[2] = {0, 0};  // cached co-resident grid size per value type (single device)
+
+template <typename VT>
+static hipError_t coop_grid(int* G) {
+    const int slot = sizeof(VT) == 4 ? 1 : 0;
+    if (g_coop_grid[slot] > 0) {
+        *G = g_coop_grid[slot];
+        return hipSuccess;
+    }
+    int dev = 0, cus = 0, per = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, jpcg_persistent<VT>, kBlock, 0);
+    if (e != hipSuccess) return e;
+    if (per < 1) return hipErrorCooperativeLaunchTooLarge;
+    g_coop_grid[slot] = cus * per;
+    *G = g_coop_grid[slot];
+    return hipSuccess;
+}
+
+hipError_t persistent_grid(bool f32, int* G) { return f32 ? coop_grid<float>(G) : coop_grid<double>(G); }
+
+hipError_t launch_jpcg(const JpcgParams& P0, bool f32, int G, hipStream_t s) {
+    JpcgParams P = P0;
+    void* args[] = {&P};
+    if (f32)
+        return hipLaunchCooperativeKernel((const void*)jpcg_persistent<float>, dim3(G), dim3(kBlock), args, 0, s);
+    return hipLaunchCooperativeKernel((const void*)jpcg_persistent<double>, dim3(G), dim3(kBlock), args, 0, s);
+}
+
+hipError_t launch_spmv(const CsrDev& A, const double* x, double* y, const double* b, hipStream_t s) {
+    if (A.ntiles <= 0) return hipSuccess;
+    if (A.val_f32)
+        spmv_tiles<float><<<A.ntiles, kBlock, 0, s>>>(A.ntiles, A.tile_row, A.rowptr, A.col,
+                                                        static_cast<const float*>(A.val), A.dsplit, x, y, b);
+    else
+        spmv_tiles<double><<<A.ntiles, kBlock, 0, s>>>(A.ntiles, A.tile_row, A.rowptr, A.col,
+                                                         static_cast<const double*>(A.val), A.dsplit, x, y, b);
+    return hipGetLastError();
+}
+
+int dot_blocks(int n) {
+    int g = (n + kBlock * 4 - 1) / (kBlock * 4);
+    if (g < 1) g = 1;
+    if (g > 2048) g = 2048;
+    return g;
+}
+
+hipError_t launch_dot(int n, const double* a, const double* b, double* part, double* out, hipStream_t s) {
+    const int g = dot_blocks(n);
+    dot_partials<<<g, kBlock, 0, s>>>(n, a, b, part);
+    sum_partials<<<1, kBlock, 0, s>>>(g, part, out);
+    return hipGetLastError();
+}
+
+static int grid_for(int64_t m) {
+    int64_t g = (m + kBlock - 1) / kBlock;
+    if (g < 1) g = 1;
+    if (g > 65536) g = 65536;
+    return (int)g;
+}
+
+hipError_t launch_gather_f64(int64_t nnz, const int64_t* perm, const double* src, double* dst, hipStream_t s) {
+    gather_values<double, double><<<grid_for(nnz), kBlock, 0, s>>>(nnz, perm, src, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_f32(int64_t nnz, const int64_t* perm, const double* src, float* dst, hipStream_t s) {
+    gather_values<float, double><<<grid_for(nnz), kBlock, 0, s>>>(nnz, perm, src, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_axpy(int n, double a, const double* d, double* x, hipStream_t s) {
+    axpy_kernel<<<grid_for(n), kBlock, 0, s>>>(n, a, d, x);
+    return hipGetLastError();
+}
+
+hipError_t launch_sddmm(const CsrDev& A, const double* lam, const double* x, double* g, hipStream_t s) {
+    if (A.ntiles <= 0) return hipSuccess;
+    sddmm_tiles<<<A.ntiles, kBlock, 0, s>>>(A.ntiles, A.tile_row, A.rowptr, A.col, lam, x, g);
+    return hipGetLastError();
+}
+
+}  // namespace xj

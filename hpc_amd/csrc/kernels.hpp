@@ -1,0 +1,166 @@
+// Device-side building blocks shared by the HIP translation units:
+// wavefront / workgroup reductions, the XCD-aware block remap, the grid
+// barrier used by the persistent kernels, and the CSR tile SpMV.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels_consts.hpp"
+
+namespace xj {
+
+constexpr int kBlock = 256;                // threads per workgroup (4 wavefronts of 64)
+constexpr int kWaves = kBlock / 64;
+constexpr int kTileNnz = kTileNnzHost;     // max stored entries per short-row tile (16 KB of fp64 products in LDS)
+constexpr int kTileRows = kTileRowsHost;   // max rows per tile: one lane reduces one row
+static_assert(kTileRows == kBlock, "one lane per tile row");
+static_assert(kTileNnz % kBlock == 0, "whole loads per lane");
+constexpr int kPerThread = kTileNnz / kBlock;
+
+// 64-byte per-iteration record, field order of the reference's CGSolverInstr
+// (pcg/hw/include/impl/cgInstr.hpp:117-132), little-endian.
+struct alignas(64) Record {
+    uint32_t id, iter, vec_size, max_iter;
+    double tols, res, rz, alpha, beta;
+    uint64_t clock;
+};
+static_assert(sizeof(Record) == 64, "record layout");
+
+// Block index -> logical index so that each XCD (workgroups are dealt to the
+// 8 XCDs round-robin) receives one contiguous range of logical blocks, keeping
+// the x-vector gathers of neighbouring row tiles in one XCD's L2.
+__device__ __forceinline__ int xcd_remap(int b, int G) {
+    const int q = G >> 3, r = G & 7;
+    const int x = b & 7, i = b >> 3;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
+// xor-butterfly: every lane ends with the same, order-fixed sum
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sum over the workgroup; every thread returns the identical value.
+// red must hold kWaves doubles of LDS.  Contains two __syncthreads.
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double s = red[0];
+#pragma unroll
+    for (int i = 1; i < kWaves; ++i) s += red[i];
+    __syncthreads();
+    return s;
+}
+
+// Sum of part[0..G) in a fixed order, identical in every workgroup.
+__device__ __forceinline__ double grid_total(const double* part, int G, double* red) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < G; i += kBlock) s += part[i];
+    return block_sum(s, red);
+}
+
+// Grid-wide barrier for cooperatively launched kernels (all workgroups
+// co-resident).  `bar` is a zeroed 64-bit arrival counter; `target` is the
+// caller's running count of expected arrivals.  A wait longer than
+// kBarrierTimeout ticks of the 100 MHz wall clock sets *abortf and every
+// workgroup leaves the barrier; returns false once the abort flag is set.
+constexpr uint64_t kBarrierTimeout = 500000000ull;  // 5 s
+
+__device__ __forceinline__ bool grid_barrier(unsigned long long* bar, unsigned long long& target, int G,
+                                             int* abortf, int* sflag) {
+    __threadfence();
+    __syncthreads();
+    target += (unsigned long long)G;
+    if (threadIdx.x == 0) {
+        atomicAdd(bar, 1ull);
+        const uint64_t t0 = wall_clock64();
+        int ab = 0;
+        while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(1);
+            if (__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { ab = 1; break; }
+            if (wall_clock64() - t0 > kBarrierTimeout) {
+                __hip_atomic_store(abortf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ab = 1;
+                break;
+            }
+        }
+        *sflag = ab;
+    }
+    __syncthreads();
+    __threadfence();
+    return *sflag == 0;
+}
+
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+    return __builtin_nontemporal_load(p);
+}
+
+// ---------------------------------------------------------------------------
+// CSR tile SpMV.
+//
+// Rows are cut into tiles on the host: a tile is either a run of consecutive
+// rows with at most kTileRows rows and kTileNnz stored entries, or one row
+// holding more than kTileNnz entries.  For a short-row tile, lanes stream the
+// tile's values and column indices contiguously (coalesced, non-temporal),
+// gather x, and park the products in LDS; then lane t sums row t's products
+// left to right.  A long row is summed by the whole workgroup with a strided
+// loop and a fixed-order workgroup reduction.
+//
+// VT is the stored value type (double, or float for the mixed-precision
+// inner operator).  With dsplit != nullptr the matrix holds no diagonal and
+// dsplit[i]*x[i] starts row i's sum.
+//
+// On return, lanes with *row >= 0 hold y[*row] in *y.
+template <typename VT>
+__device__ __forceinline__ void tile_spmv(int r0, int r1, const int* __restrict__ rowptr,
+                                          const int* __restrict__ col, const VT* __restrict__ val,
+                                          const double* __restrict__ dsplit, const double* __restrict__ x,
+                                          double* lds, double* red, double* y, int* row) {
+    const int tid = threadIdx.x;
+    const int k0 = rowptr[r0], k1 = rowptr[r1];
+    if (k1 - k0 > kTileNnz) {
+        double s = 0.0;
+        for (int k = k0 + tid; k < k1; k += kBlock) s += (double)ld_stream(val + k) * x[ld_stream(col + k)];
+        s = block_sum(s, red);
+        if (dsplit) s = dsplit[r0] * x[r0] + s;
+        *y = s;
+        *row = (tid == 0) ? r0 : -1;
+        return;
+    }
+    int c[kPerThread];
+    VT v[kPerThread];
+#pragma unroll
+    for (int i = 0; i < kPerThread; ++i) {
+        const int k = k0 + tid + i * kBlock;
+        c[i] = 0;
+        v[i] = VT(0);
+        if (k < k1) {
+            c[i] = ld_stream(col + k);
+            v[i] = ld_stream(val + k);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kPerThread; ++i) {
+        const int k = k0 + tid + i * kBlock;
+        if (k < k1) lds[k - k0] = (double)v[i] * x[c[i]];
+    }
+    __syncthreads();
+    if (tid < r1 - r0) {
+        const int a = rowptr[r0 + tid] - k0, b = rowptr[r0 + tid + 1] - k0;
+        double s = dsplit ? dsplit[r0 + tid] * x[r0 + tid] : 0.0;
+        for (int j = a; j < b; ++j) s += lds[j];
+        *y = s;
+        *row = r0 + tid;
+    } else {
+        *row = -1;
+    }
+    __syncthreads();
+}
+
+}  // namespace xj

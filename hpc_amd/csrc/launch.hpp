@@ -1,0 +1,58 @@
+// Host-side view of the device data and the kernel launchers (jpcg.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace xj {
+
+struct Record;
+
+// A CSR operator resident in HBM plus its tile plan.
+//   rowptr[n+1], col[nnz]  int32
+//   val[nnz]               fp64, or fp32 when val_f32
+//   dsplit[n]              optional fp64 diagonal kept outside the CSR
+//                          (the CSR then holds off-diagonal entries only)
+//   tile_row[ntiles+1]     first row of each tile
+struct CsrDev {
+    int n = 0;
+    int64_t nnz = 0;
+    const int* rowptr = nullptr;
+    const int* col = nullptr;
+    const void* val = nullptr;
+    bool val_f32 = false;
+    const double* dsplit = nullptr;
+    const int* tile_row = nullptr;
+    int ntiles = 0;
+};
+
+struct JpcgParams {
+    int n = 0, ntiles = 0;
+    const int* rowptr = nullptr;
+    const int* col = nullptr;
+    const void* val = nullptr;
+    const double* dsplit = nullptr;
+    const int* tile_row = nullptr;
+    const double* b = nullptr;
+    const double* diag = nullptr;  // Jacobi diagonal supplied by the caller
+    double *x = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr, *J = nullptr;
+    double* part = nullptr;              // 5*G doubles
+    unsigned long long* bar = nullptr;   // zeroed before launch
+    int* abortf = nullptr;               // zeroed before launch
+    Record* records = nullptr;           // max_rounds+1 records, or null
+    double* result = nullptr;            // [K, rr_last, dot, rz0, abort]
+    double tol = 0.0;
+    uint32_t max_rounds = 0;             // maxIter + 1, as the reference host programs it
+};
+
+hipError_t persistent_grid(bool f32, int* G);
+hipError_t launch_jpcg(const JpcgParams& P, bool f32, int G, hipStream_t s);
+hipError_t launch_spmv(const CsrDev& A, const double* x, double* y, const double* b, hipStream_t s);
+int dot_blocks(int n);
+hipError_t launch_dot(int n, const double* a, const double* b, double* part, double* out, hipStream_t s);
+hipError_t launch_gather_f64(int64_t nnz, const int64_t* perm, const double* src, double* dst, hipStream_t s);
+hipError_t launch_gather_f32(int64_t nnz, const int64_t* perm, const double* src, float* dst, hipStream_t s);
+hipError_t launch_axpy(int n, double a, const double* d, double* x, hipStream_t s);
+hipError_t launch_sddmm(const CsrDev& A, const double* lam, const double* x, double* g, hipStream_t s);
+
+}  // namespace xj

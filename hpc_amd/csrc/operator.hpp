@@ -1,0 +1,110 @@
+// C++ host engine core: a CSR operator resident in HBM, its tile plan and
+// the workspace of the persistent JPCG solver.  Shared by the pcg.h drop-in
+// (pcg_api.cpp) and the device-level C ABI (dev_api.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "launch.hpp"
+#include "pcg.h"
+
+namespace xj {
+
+// Error carrying an XJPCG status; message prefixes follow the reference's
+// exception classes (pcg/sw/include/impl/cgException.hpp:37-55).
+struct Error : std::runtime_error {
+    XJPCG_Status_t status;
+    Error(XJPCG_Status_t s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+inline Error invalid(const std::string& m) { return Error(XJPCG_STATUS_INVALID_VALUE, "INVALID VALUE ERROR: " + m); }
+inline Error internal(const std::string& m) { return Error(XJPCG_STATUS_INTERNAL_ERROR, "INTERNAL ERROR: " + m); }
+inline Error alloc_failed(const std::string& m) { return Error(XJPCG_STATUS_ALLOC_FAILED, "Alloc Failed ERROR: " + m); }
+inline Error exec_failed(const std::string& m) { return Error(XJPCG_STATUS_EXECUTION_FAILED, "Solver ERROR: " + m); }
+
+void hip_check(hipError_t e, const char* what);
+
+// Owned device allocation.
+template <typename T>
+struct DevBuf {
+    T* ptr = nullptr;
+    size_t count = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        count = 0;
+    }
+    void ensure(size_t n) {
+        if (n <= count && ptr) return;
+        release();
+        if (n == 0) n = 1;
+        hipError_t e = hipMalloc((void**)&ptr, n * sizeof(T));
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            throw alloc_failed("device allocation of " + std::to_string(n * sizeof(T)) + " bytes failed");
+        }
+        count = n;
+    }
+};
+
+// Greedy row tiling (see tile_spmv in kernels.hpp): returns tile_row[ntiles+1].
+std::vector<int32_t> plan_tiles(int32_t n, const int32_t* rowptr_host);
+
+struct SolveResult {
+    int64_t executed = 0;  // iterations run (K)
+    uint32_t iter = 0;     // reported count, min(K, maxIter)
+    double rr = 0.0;       // last recursive r.r
+    double dot = 0.0;      // b.b
+    double res = 0.0;      // sqrt(rr/dot), 0 when b == 0
+};
+
+class Operator {
+  public:
+    // References caller device memory for rowptr/col/val/dsplit (not copied);
+    // rowptr_host (n+1 entries) is used to plan tiles and may be null, in which
+    // case rowptr is copied back from the device.
+    Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t* col, const void* val, bool val_f32,
+             const double* dsplit, const int32_t* rowptr_host, hipStream_t s);
+
+    const CsrDev& csr() const { return A_; }
+    int grid() const { return G_; }
+
+    void spmv(const double* x, double* y, hipStream_t s) const;
+    // r = b - A x; returns r.r (synchronises the stream)
+    double residual(const double* b, const double* x, double* r, hipStream_t s);
+    double dot(const double* a, const double* b, int n, hipStream_t s);
+    void sddmm(const double* lam, const double* x, double* g, hipStream_t s) const;
+    // extra n-vectors owned by the operator (refinement residual and correction)
+    double* aux(int i) {
+        aux_[i].ensure((size_t)A_.n);
+        return aux_[i].ptr;
+    }
+
+    // Jacobi PCG from x0 = 0; records (device, (maxIter+2)*64 bytes) may be null.
+    SolveResult solve(const double* diag, const double* b, double* x, uint32_t max_iter, double tol, void* records,
+                      hipStream_t s);
+
+  private:
+    CsrDev A_;
+    DevBuf<int32_t> tiles_;
+    int G_ = 0;
+    DevBuf<double> r_, p_, Ap_, J_, part_, result_;
+    DevBuf<unsigned long long> bar_;
+    DevBuf<int> abort_;
+    DevBuf<double> aux_[2];
+    double* host_result_ = nullptr;  // pinned
+  public:
+    ~Operator();
+    Operator(const Operator&) = delete;
+    Operator& operator=(const Operator&) = delete;
+};
+
+}  // namespace xj

@@ -1,0 +1,521 @@
+// pcg.h drop-in: the xJPCG C API over the C++ engine core.
+//
+// Mirrors pcg/sw/src/pcg.cpp (entry points, mode dispatch, metrics, the
+// exception -> status mapping) and pcg/sw/include/impl/pcgImp.hpp (PCGImpl:
+// argument checks, matrix / vector set-up, run, result scan), with the FPGA
+// partitioner and OpenCL buffer plumbing replaced by a CSR build, HBM
+// buffers and the persistent HIP solver.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "operator.hpp"
+#include "pcg.h"
+#include "xjpcg_ext.h"
+
+namespace xj {
+namespace {
+
+using Clock = std::chrono::high_resolution_clock;
+
+double elapsed(Clock::time_point& last) {
+    auto now = Clock::now();
+    double d = std::chrono::duration<double>(now - last).count();
+    last = now;
+    return d;
+}
+
+// Errors the reference's partitioner raises as SpmException (not a
+// CgException), which pcg.cpp:112-113 maps to XJPCG_STATUS_INTERNAL_ERROR.
+Error spm_invalid(const std::string& m) { return Error(XJPCG_STATUS_INTERNAL_ERROR, "INVALID VALUE ERROR: " + m); }
+Error spm_internal(const std::string& m) { return Error(XJPCG_STATUS_INTERNAL_ERROR, "INTERNAL ERROR: " + m); }
+Error spm_not_supported(const std::string& m) {
+    return Error(XJPCG_STATUS_INTERNAL_ERROR, "Matrix Operation ERROR: " + m);
+}
+
+// Host CSR pattern with a map from CSR slot to the caller's value array.
+struct Pattern {
+    uint32_t n = 0;
+    int64_t nnz = 0;       // stored entries after expansion
+    int64_t src_len = 0;   // length of the caller's value array
+    std::vector<int32_t> rowptr, col;
+    std::vector<int64_t> perm;
+};
+
+// Bucket (row, col, src) triples by row, stably, then order each row by
+// column (ties keep input order, so duplicates stay adjacent and are summed
+// by the SpMV like the reference accumulates them).
+void finish_pattern(Pattern& P, const std::vector<uint32_t>& r, const std::vector<uint32_t>& c,
+                    const std::vector<int64_t>& src) {
+    const uint32_t n = P.n;
+    const int64_t m = (int64_t)r.size();
+    P.nnz = m;
+    P.rowptr.assign((size_t)n + 1, 0);
+    for (int64_t k = 0; k < m; ++k) P.rowptr[r[k] + 1]++;
+    for (uint32_t i = 0; i < n; ++i) P.rowptr[i + 1] += P.rowptr[i];
+    P.col.resize(m);
+    P.perm.resize(m);
+    {
+        std::vector<int64_t> pos(P.rowptr.begin(), P.rowptr.end() - 1);
+        for (int64_t k = 0; k < m; ++k) {
+            int64_t p = pos[r[k]]++;
+            P.col[p] = (int32_t)c[k];
+            P.perm[p] = src[k];
+        }
+    }
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (m < (1 << 20)) nt = 1;
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t) {
+        th.emplace_back([&, t] {
+            std::vector<std::pair<int32_t, int64_t>> tmp;
+            for (uint32_t i = (uint32_t)((uint64_t)n * t / nt); i < (uint32_t)((uint64_t)n * (t + 1) / nt); ++i) {
+                const int32_t a = P.rowptr[i], b = P.rowptr[i + 1];
+                bool sorted = true;
+                for (int32_t k = a + 1; k < b; ++k)
+                    if (P.col[k] < P.col[k - 1]) { sorted = false; break; }
+                if (sorted) continue;
+                tmp.clear();
+                for (int32_t k = a; k < b; ++k) tmp.emplace_back(P.col[k], P.perm[k]);
+                std::stable_sort(tmp.begin(), tmp.end(),
+                                 [](const std::pair<int32_t, int64_t>& x, const std::pair<int32_t, int64_t>& y) {
+                                     return x.first < y.first;
+                                 });
+                for (int32_t k = a; k < b; ++k) {
+                    P.col[k] = tmp[k - a].first;
+                    P.perm[k] = tmp[k - a].second;
+                }
+            }
+        });
+    }
+    for (auto& x : th) x.join();
+}
+
+// COO with index base (reference matrix_params.hpp:51-79 loadCoo)
+void pattern_from_coo(Pattern& P, uint32_t n, uint32_t nnz, const uint32_t* row, const uint32_t* col, int base) {
+    if (base != 0 && base != 1)
+        throw spm_not_supported("from getCooDatFromCscSym in gen_signature.hpp, unsupported storage mode.");
+    if ((int64_t)nnz >= (int64_t)INT32_MAX) throw invalid("number of non-zeros must be below 2^31.");
+    const uint32_t off = (uint32_t)base;
+    std::vector<uint32_t> r(nnz), c(nnz);
+    std::vector<int64_t> src(nnz);
+    for (uint32_t k = 0; k < nnz; ++k) {
+        uint32_t i = row[k] - off, j = col[k] - off;
+        if (i >= n || j >= n) throw invalid("matrix index out of range at entry " + std::to_string(k) + ".");
+        r[k] = i;
+        c[k] = j;
+        src[k] = k;
+    }
+    P.n = n;
+    P.src_len = nnz;
+    finish_pattern(P, r, c, src);
+}
+
+// Half-stored symmetric CSC (reference matrix_params.hpp:81-134 loadCscSym,
+// gen_signature.hpp:88-112 getCooDatFromCscSym): each off-diagonal entry is
+// emitted twice with the same value, the expansion must produce exactly
+// 2*nnz_half - n entries.
+void pattern_from_cscsym(Pattern& P, uint32_t n, uint32_t nnz_half, const int64_t* row_idx, const int64_t* col_ptr,
+                         int base) {
+    if (base != 0 && base != 1)
+        throw spm_not_supported("from getCooDatFromCscSym in gen_signature.hpp, unsupported storage mode.");
+    const int64_t off = base;
+    if (base == 1 && col_ptr[0] < 1)
+        throw spm_invalid("from getCooDatFromCscSym in gen_signature.hpp, colPtr[0] start from 1 in FORTRAN storage mode.");
+    const uint32_t full = nnz_half * 2u - n;  // uint32 arithmetic as pcgImp.hpp:87
+    if ((int64_t)full >= (int64_t)INT32_MAX) throw invalid("number of non-zeros must be below 2^31.");
+    std::vector<uint32_t> r, c;
+    std::vector<int64_t> src;
+    r.reserve(full);
+    c.reserve(full);
+    src.reserve(full);
+    for (uint32_t j = 0; j < n; ++j) {
+        for (int64_t k = col_ptr[j] - off; k < col_ptr[j + 1] - off; ++k) {
+            if (k < 0 || k >= (int64_t)nnz_half) throw spm_internal("from loadCscSym in matrix_params.hpp, index >= nnz.");
+            int64_t i = row_idx[k] - off;
+            if (i < 0 || i >= (int64_t)n) throw invalid("matrix index out of range at entry " + std::to_string(k) + ".");
+            if (r.size() >= full) throw spm_internal("from loadCscSym in matrix_params.hpp, index >= nnz.");
+            r.push_back((uint32_t)i);
+            c.push_back(j);
+            src.push_back(k);
+            if (i != (int64_t)j) {
+                if (r.size() >= full) throw spm_internal("from loadCscSym in matrix_params.hpp, index >= nnz.");
+                r.push_back(j);
+                c.push_back((uint32_t)i);
+                src.push_back(k);
+            }
+        }
+    }
+    if (r.size() != full)
+        throw spm_internal("from loadCscSym in matrix_params.hpp, failed to convert indices from CscSym to COO format.");
+    P.n = n;
+    P.src_len = nnz_half;
+    finish_pattern(P, r, c, src);
+}
+
+}  // namespace
+
+// One solver session (reference PCGImpl, pcgImp.hpp:44-253).
+class PcgHandle {
+  public:
+    XJPCG_Status_t set_status(XJPCG_Status_t s, const std::string& m) const {
+        last_status_ = s;
+        last_message_ = m;
+        return s;
+    }
+    XJPCG_Status_t last_status() const { return last_status_; }
+    const std::string& last_message() const { return last_message_; }
+    XJPCG_Metric_t* metrics() { return &metrics_; }
+    const XJPCG_Metric_t* metrics() const { return &metrics_; }
+    bool first_call() const { return first_call_; }
+
+    void init(const char* path) {
+        device_ = 0;
+        if (path != nullptr && *path != '\0') {
+            char* end = nullptr;
+            long v = std::strtol(path, &end, 10);
+            if (end != path && *end == '\0' && v >= 0) device_ = (int)v;
+        }
+        int count = 0;
+        hipError_t e = hipGetDeviceCount(&count);
+        if (e != hipSuccess || count == 0) {
+            init_error_ = "INTERNAL ERROR: no HIP device available.";
+            throw Error(XJPCG_STATUS_INTERNAL_ERROR, init_error_);
+        }
+        if (device_ >= count) {
+            init_error_ = "INVALID VALUE ERROR: HIP device " + std::to_string(device_) + " does not exist.";
+            throw Error(XJPCG_STATUS_INVALID_VALUE, init_error_);
+        }
+        hip_check(hipSetDevice(device_), "select device");
+        hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "create stream");
+        ready_ = true;
+    }
+
+    ~PcgHandle() {
+        if (ready_) {
+            (void)hipSetDevice(device_);
+            op_.reset();
+            if (stream_) (void)hipStreamDestroy(stream_);
+        }
+    }
+
+    // reference PCGImpl::setCooMat (pcgImp.hpp:49-70)
+    void set_coo(uint32_t n, uint32_t nnz, const uint32_t* row, const uint32_t* col, const double* data, int base) {
+        if (n == 0) throw invalid("Wrong dimension size.");
+        if (nnz == 0) throw invalid("Wrong non-zero element size.");
+        if (row == nullptr || col == nullptr || data == nullptr) throw invalid("Matrix is nullptr.");
+        require_device();
+        Pattern P;
+        pattern_from_coo(P, n, nnz, row, col, base);
+        upload_pattern(std::move(P), data);
+    }
+
+    // reference PCGImpl::updateMat (pcgImp.hpp:95-116)
+    void update_values(uint32_t n, uint32_t nnz, const double* data) {
+        if (n == 0) throw invalid("Wrong dimension size.");
+        if (nnz == 0) throw invalid("Wrong non-zero element size.");
+        if (data == nullptr) throw invalid("Matrix is nullptr.");
+        require_device();
+        if (!op_ || n != pat_n_ || (int64_t)nnz != pat_nnz_)
+            throw spm_invalid("Invalid dimension and/or number of nnzs used for updateing matrix.");
+        refresh_values(data, src_len_);
+    }
+
+    // reference PCGImpl::setCscSymMat (pcgImp.hpp:71-94)
+    void set_cscsym(uint32_t n, uint32_t nnz_half, const int64_t* row_idx, const int64_t* col_ptr, const double* data,
+                    int base) {
+        if (n == 0) throw invalid("Wrong dimension size.");
+        if (nnz_half == 0) throw invalid("Wrong non-zero element size.");
+        if (row_idx == nullptr || col_ptr == nullptr || data == nullptr) throw invalid("Matrix is nullptr.");
+        require_device();
+        Pattern P;
+        pattern_from_cscsym(P, n, nnz_half, row_idx, col_ptr, base);
+        upload_pattern(std::move(P), data);
+    }
+
+    // reference PCGImpl::updateCscSymMat (pcgImp.hpp:117-145)
+    void update_cscsym(uint32_t n, uint32_t nnz_half, const double* data) {
+        if (n == 0) throw invalid("Wrong dimension size.");
+        if (nnz_half == 0) throw invalid("Wrong non-zero element size.");
+        if (data == nullptr) throw invalid("Matrix is nullptr.");
+        require_device();
+        const uint32_t full = nnz_half * 2u - n;
+        if (!op_ || n != pat_n_ || (int64_t)full != pat_nnz_)
+            throw spm_invalid("Invalid dimension and/or number of nnzs used for updateing matrix.");
+        refresh_values(data, src_len_);
+    }
+
+    // reference PCGImpl::setVec (pcgImp.hpp:147-155)
+    void set_vec(uint32_t n, const double* b, const double* diag) {
+        if (b == nullptr || diag == nullptr) throw invalid("Vector is nullptr.");
+        require_device();
+        if (!op_ || n != pat_n_) throw invalid("vector size does not match the matrix.");
+        d_b_.ensure(n);
+        d_diag_.ensure(n);
+        d_x_.ensure(n);
+        hip_check(hipMemcpyAsync(d_b_.ptr, b, sizeof(double) * n, hipMemcpyHostToDevice, stream_), "upload b");
+        hip_check(hipMemcpyAsync(d_diag_.ptr, diag, sizeof(double) * n, hipMemcpyHostToDevice, stream_),
+                  "upload diagonal");
+        hip_check(hipStreamSynchronize(stream_), "upload vectors");
+    }
+
+    // reference PCGImpl::run (pcgImp.hpp:157-183): solve, then copy x out
+    SolveResult run(uint32_t max_iter, double tol, double* x_out) {
+        require_device();
+        void* rec = nullptr;
+        const uint64_t nrec = (uint64_t)max_iter + 2;
+        if (nrec <= kMaxRecords) {
+            d_records_.ensure(nrec * 64);
+            rec = d_records_.ptr;
+        }
+        SolveResult R = op_->solve(d_diag_.ptr, d_b_.ptr, d_x_.ptr, max_iter, tol, rec, stream_);
+        hip_check(hipMemcpyAsync(x_out, d_x_.ptr, sizeof(double) * pat_n_, hipMemcpyDeviceToHost, stream_),
+                  "copy x");
+        hip_check(hipStreamSynchronize(stream_), "copy x");
+        records_valid_ = rec != nullptr;
+        last_executed_ = R.executed;
+        first_call_ = false;
+        return R;
+    }
+
+    // records 0..K of the last solve
+    uint64_t history(void* out, uint64_t capacity) {
+        if (!records_valid_) return 0;
+        uint64_t count = (uint64_t)last_executed_ + 1;
+        uint64_t m = std::min(count, capacity);
+        if (out && m) {
+            hip_check(hipMemcpyAsync(out, d_records_.ptr, m * 64, hipMemcpyDeviceToHost, stream_), "copy records");
+            hip_check(hipStreamSynchronize(stream_), "copy records");
+        }
+        return count;
+    }
+
+  private:
+    static constexpr uint64_t kMaxRecords = 1ull << 20;  // 64 MiB of records at most
+
+    void require_device() {
+        if (!ready_) throw Error(XJPCG_STATUS_INTERNAL_ERROR, init_error_.empty() ? "INTERNAL ERROR: handle not initialised." : init_error_);
+        hip_check(hipSetDevice(device_), "select device");
+    }
+
+    void upload_pattern(Pattern&& P, const double* data) {
+        op_.reset();
+        const uint32_t n = P.n;
+        d_rowptr_.ensure((size_t)n + 1);
+        d_col_.ensure((size_t)P.nnz);
+        d_val_.ensure((size_t)P.nnz);
+        d_perm_.ensure((size_t)P.nnz);
+        hip_check(hipMemcpyAsync(d_rowptr_.ptr, P.rowptr.data(), sizeof(int32_t) * ((size_t)n + 1),
+                                 hipMemcpyHostToDevice, stream_), "upload row pointers");
+        hip_check(hipMemcpyAsync(d_col_.ptr, P.col.data(), sizeof(int32_t) * (size_t)P.nnz, hipMemcpyHostToDevice,
+                                 stream_), "upload columns");
+        hip_check(hipMemcpyAsync(d_perm_.ptr, P.perm.data(), sizeof(int64_t) * (size_t)P.nnz, hipMemcpyHostToDevice,
+                                 stream_), "upload value map");
+        pat_n_ = n;
+        pat_nnz_ = P.nnz;
+        src_len_ = P.src_len;
+        refresh_values(data, P.src_len);
+        op_.reset(new Operator((int32_t)n, P.nnz, d_rowptr_.ptr, d_col_.ptr, d_val_.ptr, false, nullptr,
+                               P.rowptr.data(), stream_));
+    }
+
+    // values reach their CSR slots on the GPU: val[k] = data[perm[k]]
+    void refresh_values(const double* data, int64_t len) {
+        d_raw_.ensure((size_t)len);
+        hip_check(hipMemcpyAsync(d_raw_.ptr, data, sizeof(double) * (size_t)len, hipMemcpyHostToDevice, stream_),
+                  "upload values");
+        hip_check(launch_gather_f64(pat_nnz_, d_perm_.ptr, d_raw_.ptr, d_val_.ptr, stream_), "place values");
+        hip_check(hipStreamSynchronize(stream_), "place values");
+    }
+
+    mutable XJPCG_Status_t last_status_ = XJPCG_STATUS_SUCCESS;
+    mutable std::string last_message_;
+    XJPCG_Metric_t metrics_{0.0, 0.0, 0.0, 0.0};
+    bool first_call_ = true;
+    bool ready_ = false;
+    std::string init_error_;
+    int device_ = 0;
+    hipStream_t stream_ = nullptr;
+
+    uint32_t pat_n_ = 0;
+    int64_t pat_nnz_ = 0, src_len_ = 0;
+    DevBuf<int32_t> d_rowptr_, d_col_;
+    DevBuf<double> d_val_, d_raw_;
+    DevBuf<int64_t> d_perm_;
+    DevBuf<double> d_b_, d_diag_, d_x_;
+    DevBuf<uint8_t> d_records_;
+    bool records_valid_ = false;
+    int64_t last_executed_ = 0;
+    std::unique_ptr<Operator> op_;
+};
+
+}  // namespace xj
+
+using xj::PcgHandle;
+
+namespace {
+
+PcgHandle* as_impl(XJPCG_Handle_t* h) { return reinterpret_cast<PcgHandle*>(h); }
+const PcgHandle* as_impl(const XJPCG_Handle_t* h) { return reinterpret_cast<const PcgHandle*>(h); }
+
+// catch blocks of pcg.cpp:46-50 / :110-114 / :167-171
+template <typename F>
+XJPCG_Status_t run_guarded(const PcgHandle* h, F f, const char* ok_message) {
+    try {
+        f();
+    } catch (const xj::Error& e) {
+        return h->set_status(e.status, e.what());
+    } catch (const std::exception& e) {
+        return h->set_status(XJPCG_STATUS_INTERNAL_ERROR, e.what());
+    }
+    return h->set_status(XJPCG_STATUS_SUCCESS, ok_message);
+}
+
+}  // namespace
+
+extern "C" {
+
+// reference pcg.cpp:34-52
+XJPCG_Status_t xJPCG_createHandle(XJPCG_Handle_t** handle, const char* xclbinPath) {
+    if (handle == nullptr) return XJPCG_STATUS_INVALID_VALUE;
+    PcgHandle* h = new PcgHandle();
+    *handle = reinterpret_cast<XJPCG_Handle_t*>(h);
+    return run_guarded(h, [&] {
+        auto last = xj::Clock::now();
+        h->init(xclbinPath);
+        h->metrics()->m_objInit = xj::elapsed(last);
+    }, "Handle is created successfully");
+}
+
+// reference pcg.cpp:54-59
+XJPCG_Status_t xJPCG_destroyHandle(XJPCG_Handle_t* handle) {
+    if (handle == nullptr) return XJPCG_STATUS_NOT_INITIALIZED;
+    delete as_impl(handle);
+    return XJPCG_STATUS_SUCCESS;
+}
+
+// reference pcg.cpp:61-116
+XJPCG_Status_t xJPCG_cscSymSolver(XJPCG_Handle_t* handle, const int64_t p_n, const int64_t p_nnz,
+                                  const int64_t* p_rowIdx, const int64_t* p_colPtr, const double* p_data,
+                                  const double* p_diagA, const double* p_b, const double* p_x,
+                                  const int64_t p_maxIter, const double p_tol, int64_t* p_iter, double* p_res,
+                                  const XJPCG_Mode_t mode) {
+    if (handle == nullptr) return XJPCG_STATUS_NOT_INITIALIZED;
+    PcgHandle* h = as_impl(handle);
+    return run_guarded(h, [&] {
+        auto last = xj::Clock::now();
+        const bool first = h->first_call();
+        const uint32_t n = (uint32_t)p_n, nnz = (uint32_t)p_nnz;  // PCGImpl takes uint32 sizes
+        switch (mode & 0x0f) {
+            case XJPCG_MODE_DEFAULT:
+                h->set_cscsym(n, nnz, p_rowIdx, p_colPtr, p_data, (mode & 0xf0) >> 4);
+                break;
+            case XJPCG_MODE_KEEP_NZ_LAYOUT:
+                if (first) throw xj::invalid("wrong solver mode for the first call, please use XJPCG_MODEL_DEFAULT.");
+                h->update_cscsym(n, nnz, p_data);
+                break;
+            default:
+                if (first) throw xj::invalid("wrong solver mode for the first call, please use XJPCG_MODEL_DEFAULT.");
+                break;
+        }
+        h->metrics()->m_matProc = xj::elapsed(last);
+        h->set_vec(n, p_b, p_diagA);
+        h->metrics()->m_vecProc = xj::elapsed(last);
+        if (p_iter == nullptr || p_res == nullptr || p_x == nullptr) throw xj::invalid("output pointer is nullptr.");
+        xj::SolveResult R = h->run((uint32_t)p_maxIter, p_tol, const_cast<double*>(p_x));
+        *p_res = R.res;
+        *p_iter = R.iter;
+        h->metrics()->m_solver = xj::elapsed(last);
+        if (*p_res > p_tol)
+            throw xj::exec_failed("exit with divergent solution after " + std::to_string(*p_iter) + " iterations.");
+    }, "Solver returns successfully");
+}
+
+// reference pcg.cpp:118-173
+XJPCG_Status_t xJPCG_cooSolver(XJPCG_Handle_t* handle, const uint32_t p_n, const uint32_t p_nnz,
+                               const uint32_t* p_rowIdx, const uint32_t* p_colIdx, const double* p_data,
+                               const double* matJ, const double* b, const double* x, const uint32_t p_maxIter,
+                               const double p_tol, uint32_t* p_iter, double* p_res, const XJPCG_Mode_t mode) {
+    if (handle == nullptr) return XJPCG_STATUS_NOT_INITIALIZED;
+    PcgHandle* h = as_impl(handle);
+    return run_guarded(h, [&] {
+        auto last = xj::Clock::now();
+        const bool first = h->first_call();
+        switch (mode & 0x0f) {
+            case XJPCG_MODE_DEFAULT:
+                h->set_coo(p_n, p_nnz, p_rowIdx, p_colIdx, p_data, (mode & 0xf0) >> 4);
+                break;
+            case XJPCG_MODE_KEEP_NZ_LAYOUT:
+                if (first) throw xj::invalid("wrong solver mode for the first call, please use XJPCG_MODEL_DEFAULT.");
+                h->update_values(p_n, p_nnz, p_data);
+                break;
+            default:
+                if (first) throw xj::invalid("wrong solver mode for the first call, please use XJPCG_MODEL_DEFAULT.");
+                break;
+        }
+        h->metrics()->m_matProc = xj::elapsed(last);
+        h->set_vec(p_n, b, matJ);
+        h->metrics()->m_vecProc = xj::elapsed(last);
+        if (p_iter == nullptr || p_res == nullptr || x == nullptr) throw xj::invalid("output pointer is nullptr.");
+        xj::SolveResult R = h->run(p_maxIter, p_tol, const_cast<double*>(x));
+        *p_res = R.res;
+        *p_iter = R.iter;
+        h->metrics()->m_solver = xj::elapsed(last);
+        if (*p_res > p_tol)
+            throw xj::exec_failed("exit with divergent solution after " + std::to_string(*p_iter) + " iterations.");
+    }, "Solver returns successfully");
+}
+
+// reference pcg.cpp:175-180
+XJPCG_Status_t xJPCG_getMetrics(const XJPCG_Handle_t* handle, XJPCG_Metric_t* metric) {
+    if (handle == nullptr) return XJPCG_STATUS_NOT_INITIALIZED;
+    const PcgHandle* h = as_impl(handle);
+    if (metric == nullptr) return h->set_status(XJPCG_STATUS_INVALID_VALUE, "INVALID VALUE ERROR: metric is nullptr.");
+    *metric = *h->metrics();
+    return h->set_status(XJPCG_STATUS_SUCCESS, "Get metrics successfully");
+}
+
+// reference pcg.cpp:182-188
+XJPCG_Status_t xJPCG_peekAtLastStatus(const XJPCG_Handle_t* handle) {
+    if (handle == nullptr) return XJPCG_STATUS_NOT_INITIALIZED;
+    return as_impl(handle)->last_status();
+}
+
+// reference pcg.cpp:190-194
+const char* xJPCG_getLastMessage(const XJPCG_Handle_t* handle) {
+    if (handle == nullptr) return "Can't retrieve last error message from a null JPCG handle.";
+    return as_impl(handle)->last_message().c_str();
+}
+
+// reference pcg.cpp:196-214
+const char* xJPCG_getErrorString(const XJPCG_Status_t code) {
+    switch (code) {
+        case XJPCG_STATUS_SUCCESS: return "XJPCG_STATUS_SUCCESS";
+        case XJPCG_STATUS_NOT_INITIALIZED: return "XJPCG_STATUS_NOT_INITIALIZED";
+        case XJPCG_STATUS_ALLOC_FAILED: return "XJPCG_STATUS_ALLOC_FAILED";
+        case XJPCG_STATUS_INVALID_VALUE: return "XJPCG_STATUS_INVALID_VALUE";
+        case XJPCG_STATUS_EXECUTION_FAILED: return "XJPCG_STATUS_EXECUTION_FAILED";
+        case XJPCG_STATUS_INTERNAL_ERROR: return "XJPCG_STATUS_INTERNAL_ERROR";
+        case XJPCG_STATUS_DYNAMIC_LOADING_ERROR: return "XJPCG_STATUS_DYNAMIC_LOADING_ERROR";
+    }
+    return "Unknown Error Code!";
+}
+
+// per-iteration records of the last solve (extension, include/xjpcg_ext.h)
+XJPCG_Status_t xJPCG_getHistory(const XJPCG_Handle_t* handle, void* records, uint64_t capacity, uint64_t* count) {
+    if (handle == nullptr) return XJPCG_STATUS_NOT_INITIALIZED;
+    PcgHandle* h = const_cast<PcgHandle*>(as_impl(handle));
+    return run_guarded(h, [&] {
+        if (count == nullptr) throw xj::invalid("count is nullptr.");
+        *count = h->history(records, capacity);
+    }, "Get history successfully");
+}
+
+}  // extern "C"

@@ -1,0 +1,171 @@
+"""Device-level solver objects over include/xjpcg_dev.h, on torch tensors in HBM.
+
+torch provides the device memory and the stream; every computation runs in
+the HIP kernels of libxjpcg.so.  Calls launch on torch's current stream, so
+torch.cuda.Event timing on that stream brackets the library's kernels.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import torch
+
+from ._lib import check_dev, lib, ptr
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dev_check(t, dtype, name, n=None):
+    if not (t.is_cuda and t.dtype == dtype and t.is_contiguous()):
+        raise ValueError(f"{name} must be a contiguous {dtype} CUDA tensor")
+    if n is not None and t.numel() != n:
+        raise ValueError(f"{name} must have {n} entries, has {t.numel()}")
+
+
+@dataclass
+class SolveInfo:
+    iterations: int       # reported count, min(executed, maxIter)
+    executed: int         # iterations run
+    residual: float       # ||r_recursive|| / ||b||
+
+
+@dataclass
+class MixedInfo:
+    outer: int            # refinement steps
+    inner_total: int      # inner JPCG iterations over all steps
+    residual: float       # true ||b - A x|| / ||b|| in fp64
+
+
+class CsrOperator:
+    """CSR operator in HBM (int32 rowptr/col, fp64 or fp32 values, optional split fp64 diagonal).
+
+    The tensors are referenced, not copied; the operator keeps them alive.
+    """
+
+    def __init__(self, rowptr: torch.Tensor, col: torch.Tensor, val: torch.Tensor, dsplit: torch.Tensor | None = None,
+                 rowptr_host=None):
+        n = rowptr.numel() - 1
+        _dev_check(rowptr, torch.int32, "rowptr")
+        _dev_check(col, torch.int32, "col")
+        if val.dtype not in (torch.float64, torch.float32):
+            raise ValueError("val must be float64 or float32")
+        _dev_check(val, val.dtype, "val", col.numel())
+        if dsplit is not None:
+            _dev_check(dsplit, torch.float64, "dsplit", n)
+        self.n = n
+        self.nnz = col.numel()
+        self.rowptr, self.col, self.val, self.dsplit = rowptr, col, val, dsplit
+        self.f32 = val.dtype == torch.float32
+        h = ctypes.c_void_p()
+        hp = None
+        if rowptr_host is not None:
+            import numpy as np
+            self._rp_host = np.ascontiguousarray(rowptr_host, dtype=np.int32)
+            hp = self._rp_host.ctypes.data
+        check_dev(lib().xjpcg_op_create(ctypes.byref(h), n, self.nnz, ptr(rowptr), ptr(col), ptr(val),
+                                        1 if self.f32 else 0, ptr(dsplit), hp, _stream()))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().xjpcg_op_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        n, nnz, nt, g = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
+        check_dev(lib().xjpcg_op_info(self._h, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(nt),
+                                      ctypes.byref(g)))
+        return {"n": n.value, "nnz": nnz.value, "ntiles": nt.value, "grid": g.value}
+
+    def spmv(self, x: torch.Tensor, y: torch.Tensor | None = None) -> torch.Tensor:
+        _dev_check(x, torch.float64, "x", self.n)
+        if y is None:
+            y = torch.empty_like(x)
+        check_dev(lib().xjpcg_spmv(self._h, ptr(x), ptr(y), _stream()))
+        return y
+
+    def residual(self, b: torch.Tensor, x: torch.Tensor, r: torch.Tensor | None = None):
+        _dev_check(b, torch.float64, "b", self.n)
+        _dev_check(x, torch.float64, "x", self.n)
+        if r is None:
+            r = torch.empty_like(b)
+        rr = ctypes.c_double()
+        check_dev(lib().xjpcg_residual(self._h, ptr(b), ptr(x), ptr(r), ctypes.byref(rr), _stream()))
+        return r, rr.value
+
+    def solve(self, diag: torch.Tensor, b: torch.Tensor, max_iter: int, tol: float, x: torch.Tensor | None = None,
+              records: torch.Tensor | None = None) -> tuple[torch.Tensor, SolveInfo]:
+        """Jacobi PCG from x0 = 0 (reference semantics); records: uint8 tensor of (max_iter+2)*64 bytes."""
+        _dev_check(diag, torch.float64, "diag", self.n)
+        _dev_check(b, torch.float64, "b", self.n)
+        if x is None:
+            x = torch.empty_like(b)
+        _dev_check(x, torch.float64, "x", self.n)
+        if records is not None:
+            _dev_check(records, torch.uint8, "records", (int(max_iter) + 2) * 64)
+        it, res, ex = ctypes.c_uint32(), ctypes.c_double(), ctypes.c_int64()
+        check_dev(lib().xjpcg_solve(self._h, ptr(diag), ptr(b), ptr(x), int(max_iter), float(tol), ctypes.byref(it),
+                                    ctypes.byref(res), ctypes.byref(ex), ptr(records), _stream()))
+        return x, SolveInfo(it.value, ex.value, res.value)
+
+    def sddmm(self, lam: torch.Tensor, x: torch.Tensor, g: torch.Tensor | None = None) -> torch.Tensor:
+        _dev_check(lam, torch.float64, "lam", self.n)
+        _dev_check(x, torch.float64, "x", self.n)
+        if g is None:
+            g = torch.empty(self.nnz, dtype=torch.float64, device=x.device)
+        check_dev(lib().xjpcg_sddmm(self._h, ptr(lam), ptr(x), ptr(g), _stream()))
+        return g
+
+
+def solve_mixed(outer: CsrOperator, inner: CsrOperator, diag: torch.Tensor, b: torch.Tensor, tol: float,
+                inner_tol: float = 1e-6, inner_max_iter: int = 20000, max_outer: int = 30,
+                x: torch.Tensor | None = None, raise_on_fail: bool = True) -> tuple[torch.Tensor, MixedInfo]:
+    """fp32-valued inner JPCG inside an fp64 iterative-refinement loop."""
+    if not inner.f32:
+        raise ValueError("inner operator must hold fp32 values")
+    _dev_check(diag, torch.float64, "diag", outer.n)
+    _dev_check(b, torch.float64, "b", outer.n)
+    if x is None:
+        x = torch.empty_like(b)
+    no, it, res = ctypes.c_uint32(), ctypes.c_int64(), ctypes.c_double()
+    st = lib().xjpcg_solve_mixed(outer.handle, inner.handle, ptr(diag), ptr(b), ptr(x), int(max_outer), float(tol),
+                                 float(inner_tol), int(inner_max_iter), ctypes.byref(no), ctypes.byref(it),
+                                 ctypes.byref(res), _stream())
+    if st != 0 and (raise_on_fail or st != 4):
+        check_dev(st)
+    return x, MixedInfo(no.value, it.value, res.value)
+
+
+def split_fp32(rowptr_h, col_h, val_h):
+    """Host helper: fp32 off-diagonal CSR + fp64 diagonal from an fp64 CSR (numpy arrays).
+
+    Returns (rowptr, col, val32, diag64, exact) where exact tells whether every
+    off-diagonal value is representable in fp32 (then the split operator is the
+    fp64 operator exactly).
+    """
+    import numpy as np
+    n = rowptr_h.size - 1
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(rowptr_h))
+    is_d = col_h == rows
+    diag = np.zeros(n, dtype=np.float64)
+    np.add.at(diag, rows[is_d], val_h[is_d])
+    keep = ~is_d
+    cnt = np.bincount(rows[keep], minlength=n)
+    rp = np.zeros(n + 1, dtype=np.int32)
+    np.cumsum(cnt, out=rp[1:])
+    v32 = val_h[keep].astype(np.float32)
+    exact = bool(np.all(v32.astype(np.float64) == val_h[keep]))
+    return rp, np.ascontiguousarray(col_h[keep], dtype=np.int32), v32, diag, exact

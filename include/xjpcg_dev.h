@@ -1,0 +1,90 @@
+/*
+ * Device-level C ABI of libxjpcg.so: the thin layer between host code (the
+ * pcg.h engine, the Python package, the PyTorch autograd function) and the
+ * HIP kernels.  Arguments are plain pointers into HBM (d_*), host pointers
+ * (h_*) and sizes; `stream` is a hipStream_t passed as void* (NULL = the null
+ * stream).  Calls return an XJPCG_Status_t value; xjpcg_dev_last_error()
+ * describes the last failure on the calling thread.
+ *
+ * These entry points split the reference's PCGImpl host operations
+ * (pcg/sw/include/impl/pcgImp.hpp:49-204) -- matrix load, vector load, run,
+ * iteration / residual query, result retrieval -- over device buffers, and
+ * add three forms with no reference counterpart: the mixed-precision refined
+ * solve, the batched solve and the SDDMM gradient.
+ *
+ * Operators reference the caller's device memory (nothing is copied); the
+ * caller keeps it alive until xjpcg_op_destroy.  Not thread-safe per operator.
+ */
+#ifndef XJPCG_DEV_H
+#define XJPCG_DEV_H
+
+#include <stdint.h>
+
+#include "pcg.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct xjpcg_op xjpcg_op_t;
+
+/* message of the last failing call on this thread ("" if none) */
+const char* xjpcg_dev_last_error(void);
+
+/* CSR operator A (n x n): d_rowptr[n+1], d_col[nnz] int32; d_val[nnz] fp64
+ * (val_f32 = 0) or fp32 (val_f32 = 1).  With d_dsplit != NULL the CSR holds
+ * only off-diagonal entries and d_dsplit[n] (fp64) is the diagonal.
+ * h_rowptr (n+1 host entries) may be NULL; the row pointers are then read
+ * back from the device to plan the row tiles.
+ * Replaces the matrix-load step PCGImpl::setCooMat (pcgImp.hpp:49-70). */
+XJPCG_Status_t xjpcg_op_create(xjpcg_op_t** op, int32_t n, int64_t nnz, const int32_t* d_rowptr,
+                               const int32_t* d_col, const void* d_val, int val_f32, const double* d_dsplit,
+                               const int32_t* h_rowptr, void* stream);
+XJPCG_Status_t xjpcg_op_destroy(xjpcg_op_t* op);
+/* n, nnz, number of row tiles, grid of the persistent kernel */
+XJPCG_Status_t xjpcg_op_info(const xjpcg_op_t* op, int32_t* n, int64_t* nnz, int32_t* ntiles, int32_t* grid);
+
+/* d_y = A d_x  (the reference's SpMV compute units, L2/sparse/src/hw/fp64) */
+XJPCG_Status_t xjpcg_spmv(xjpcg_op_t* op, const double* d_x, double* d_y, void* stream);
+
+/* d_r = d_b - A d_x and *h_rr = r.r (synchronises the stream) */
+XJPCG_Status_t xjpcg_residual(xjpcg_op_t* op, const double* d_b, const double* d_x, double* d_r, double* h_rr,
+                              void* stream);
+
+/* Jacobi-preconditioned CG from x0 = 0 in one persistent launch, semantics of
+ * the reference device loop and result scan (control.hpp:65-91,
+ * pcgImp.hpp:157-183, pcg.cpp:159-160): at most maxIter+1 iterations,
+ * *h_iter = min(executed, maxIter), *h_res = ||r||/||b||; *h_executed (may be
+ * NULL) = iterations actually run.  d_records, if not NULL, receives
+ * (maxIter+2) 64-byte records laid out as the reference's CGSolverInstr
+ * (cgInstr.hpp:117-132): record 0 holds the set-up, record k iteration k.
+ * Returns SUCCESS whether or not the tolerance was met (compare *h_res). */
+XJPCG_Status_t xjpcg_solve(xjpcg_op_t* op, const double* d_diag, const double* d_b, double* d_x, uint32_t max_iter,
+                           double tol, uint32_t* h_iter, double* h_res, int64_t* h_executed, void* d_records,
+                           void* stream);
+
+/* d_g[k] = -d_lam[row(k)] * d_x[col(k)] for every stored entry k: the
+ * gradient of a loss through x = A^-1 b with respect to A's stored values,
+ * given the adjoint solution lam = A^-T dL/dx. */
+XJPCG_Status_t xjpcg_sddmm(xjpcg_op_t* op, const double* d_lam, const double* d_x, double* d_g, void* stream);
+
+/* Mixed-precision iterative refinement.
+ *   outer: fp64-accurate operator for the true residual r = b - A x (fp64
+ *          values, or fp32 off-diagonal values with an fp64 split diagonal
+ *          when those values are exact in fp32);
+ *   inner: fp32-valued operator solved by the persistent JPCG (fp64 vectors,
+ *          fp64 accumulation, fp64 alpha/beta dot products).
+ * Loop: r = b - A x (fp64); stop when ||r||/||b|| < tol; solve inner d = r to
+ * relative tolerance inner_tol (at most inner_max_iter iterations); x += d.
+ * d_x receives the solution.  Counts and the final true residual are returned
+ * through the h_ pointers (any may be NULL).  Returns EXECUTION_FAILED when
+ * tol is not reached within max_outer refinement steps. */
+XJPCG_Status_t xjpcg_solve_mixed(xjpcg_op_t* outer, xjpcg_op_t* inner, const double* d_diag, const double* d_b,
+                                 double* d_x, uint32_t max_outer, double tol, double inner_tol,
+                                 uint32_t inner_max_iter, uint32_t* h_outer, int64_t* h_inner_total,
+                                 double* h_res, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

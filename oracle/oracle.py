@@ -1,0 +1,154 @@
+"""ctypes binding of oracle/liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as
+the checker / CPU baseline.  The product package (hpc_amd/) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libxjpcg_ref.so")
+
+_L = None
+_R = None
+
+
+def build(ref: bool = False):
+    targets = ["all"] + (["ref"] if ref else [])
+    subprocess.run(["make", "-s", "-C", HERE] + targets, check=True)
+
+
+def lib():
+    global _L
+    if _L is None:
+        if not os.path.exists(LIB):
+            build()
+        _L = ctypes.CDLL(LIB)
+        v, i32, i64, u32, d = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint32, ctypes.c_double
+        _L.orc_coo_to_csr.argtypes = [u32, u32, v, v, v, ctypes.c_int, v, v, v, v]
+        _L.orc_cscsym_to_coo.restype = i64
+        _L.orc_cscsym_to_coo.argtypes = [i64, i64, v, v, v, ctypes.c_int, v, v, v]
+        _L.orc_spmv.argtypes = [i32, v, v, v, v, v]
+        _L.orc_spmv_omp.argtypes = [i32, v, v, v, v, v, ctypes.c_int]
+        _L.orc_dot_fixed.restype = d
+        _L.orc_dot_fixed.argtypes = [i64, v, v]
+        _L.orc_cg_init.argtypes = [u32, v, v, ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(u32)]
+        _L.orc_record_encode.argtypes = [v, v]
+        _L.orc_record_decode.argtypes = [v, v]
+        _L.orc_jpcg.argtypes = [i32, v, v, v, v, v, u32, d, v, ctypes.POINTER(u32), ctypes.POINTER(d), v, v,
+                                ctypes.c_int]
+    return _L
+
+
+def ref_lib():
+    """oracle/_ref/libxjpcg_ref.so (reference host code), or None where it cannot be built."""
+    global _R
+    if _R is None:
+        if not os.path.exists(REF_LIB):
+            try:
+                build(ref=True)
+            except Exception:
+                return None
+        if not os.path.exists(REF_LIB):
+            return None
+        _R = ctypes.CDLL(REF_LIB)
+    return _R
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def coo_to_csr(n, row, col, val, base=0):
+    row = np.ascontiguousarray(row, np.uint32)
+    col = np.ascontiguousarray(col, np.uint32)
+    val = np.ascontiguousarray(val, np.float64)
+    rp = np.zeros(n + 1, np.int32)
+    c = np.zeros(row.size, np.int32)
+    v = np.zeros(row.size, np.float64)
+    perm = np.zeros(row.size, np.int64)
+    rc = lib().orc_coo_to_csr(n, row.size, _p(row), _p(col), _p(val), base, _p(rp), _p(c), _p(v), _p(perm))
+    if rc != 0:
+        raise ValueError("bad COO input")
+    return rp, c, v, perm
+
+
+def cscsym_to_coo(n, row_idx, col_ptr, data, base=0):
+    row_idx = np.ascontiguousarray(row_idx, np.int64)
+    col_ptr = np.ascontiguousarray(col_ptr, np.int64)
+    data = np.ascontiguousarray(data, np.float64)
+    full = 2 * data.size - n
+    r = np.zeros(full, np.uint32)
+    c = np.zeros(full, np.uint32)
+    v = np.zeros(full, np.float64)
+    m = lib().orc_cscsym_to_coo(n, data.size, _p(row_idx), _p(col_ptr), _p(data), base, _p(r), _p(c), _p(v))
+    if m < 0:
+        raise ValueError("bad CSC-sym input")
+    return r, c, v
+
+
+def spmv(rowptr, col, val, x, threads=1):
+    n = rowptr.size - 1
+    y = np.zeros(n, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    if threads > 1:
+        lib().orc_spmv_omp(n, _p(rowptr), _p(col), _p(val), _p(x), _p(y), threads)
+    else:
+        lib().orc_spmv(n, _p(rowptr), _p(col), _p(val), _p(x), _p(y))
+    return y
+
+
+def dot_fixed(a, b):
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    return lib().orc_dot_fixed(a.size, _p(a), _p(b))
+
+
+def cg_init(b, diag):
+    b = np.ascontiguousarray(b, np.float64)
+    diag = np.ascontiguousarray(diag, np.float64)
+    dot, rz, da = ctypes.c_double(), ctypes.c_double(), ctypes.c_uint32()
+    lib().orc_cg_init(b.size, _p(b), _p(diag), ctypes.byref(dot), ctypes.byref(rz), ctypes.byref(da))
+    return dot.value, rz.value, da.value
+
+
+RECORD_DTYPE = np.dtype([("id", "<u4"), ("iter", "<u4"), ("vec_size", "<u4"), ("max_iter", "<u4"),
+                         ("tols", "<f8"), ("res", "<f8"), ("rz", "<f8"), ("alpha", "<f8"), ("beta", "<f8"),
+                         ("clock", "<u8")])
+
+
+def record_encode(rec: dict) -> bytes:
+    a = np.zeros(1, RECORD_DTYPE)
+    for k, v in rec.items():
+        a[k] = v
+    out = np.zeros(64, np.uint8)
+    lib().orc_record_encode(a.ctypes.data, out.ctypes.data)
+    return out.tobytes()
+
+
+def jpcg(rowptr, col, val, diag, b, max_iter, tol, threads=1, records=False):
+    """Returns dict(x, iter, res, executed, rr_hist, records)."""
+    n = rowptr.size - 1
+    rowptr = np.ascontiguousarray(rowptr, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    diag = np.ascontiguousarray(diag, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(n, np.float64)
+    it, res = ctypes.c_uint32(), ctypes.c_double()
+    rec = np.zeros((max_iter + 2) * 64, np.uint8) if records else None
+    hist = np.zeros(max_iter + 1, np.float64)
+    k = lib().orc_jpcg(n, _p(rowptr), _p(col), _p(val), _p(diag), _p(b), max_iter, tol, _p(x), ctypes.byref(it),
+                       ctypes.byref(res), _p(rec), _p(hist), threads)
+    if k < 0:
+        raise MemoryError("oracle allocation failed")
+    out = {"x": x, "iter": it.value, "res": res.value, "executed": k, "rr_hist": hist[:k]}
+    if records:
+        out["records"] = rec.view(RECORD_DTYPE)
+    return out

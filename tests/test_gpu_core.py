@@ -1,0 +1,290 @@
+"""GPU parity of the SpMV and the persistent JPCG against the CPU oracle (through the C ABI)."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from hpc_amd import workloads
+from oracle import oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def to_dev(cuda, A):
+    import torch
+    from hpc_amd import device
+    rp = torch.from_numpy(np.ascontiguousarray(A.rowptr, np.int32)).to(cuda)
+    col = torch.from_numpy(np.ascontiguousarray(A.col, np.int32)).to(cuda)
+    val = torch.from_numpy(np.ascontiguousarray(A.val, np.float64)).to(cuda)
+    return device.CsrOperator(rp, col, val, rowptr_host=A.rowptr)
+
+
+def dvec(cuda, a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(cuda)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+# ----------------------------------------------------------------------------- SpMV
+
+@pytest.mark.parametrize("case", [0, 1])
+def test_spmv_golden_cases(cuda, case):
+    z = np.load(os.path.join(GOLDEN, "spmv_cases.npz"))
+    n = int(z[f"n{case}"])
+    rp, col, val, _ = oracle.coo_to_csr(n, z[f"row{case}"], z[f"col{case}"], z[f"val{case}"])
+    _say(f"spmv golden case {case}: n={n} nnz={col.size}")
+    A = workloads.CsrMatrix(rp, col, val, np.ones(n))
+    op = to_dev(cuda, A)
+    y = op.spmv(dvec(cuda, z[f"x{case}"])).cpu().numpy()
+    assert np.allclose(y, z[f"y{case}"], rtol=1e-13, atol=1e-13)
+    y_orc = oracle.spmv(rp, col, val, z[f"x{case}"])
+    if case == 0:  # short rows: same products, same left-to-right row sums => bit-exact
+        assert np.array_equal(y, y_orc)
+    else:
+        assert np.allclose(y, y_orc, rtol=1e-13, atol=1e-12)
+
+
+@pytest.mark.parametrize("kind,grid", [(0, (64, 48, 1)), (1, (20, 17, 9)), (2, (24, 24, 24))])
+def test_spmv_stencil_bitexact(cuda, kind, grid):
+    coef = {0: [1.0, 1.7], 1: [0.5, 1.25, 2.0], 2: [26.0, -1.0]}[kind]
+    A = workloads.stencil(kind, *grid, coef)
+    _say(f"spmv stencil kind={kind} grid={grid} nnz={A.nnz}")
+    x = workloads.uniform(9, 0, A.n, -1, 1)
+    y = to_dev(cuda, A).spmv(dvec(cuda, x)).cpu().numpy()
+    assert np.array_equal(y, oracle.spmv(A.rowptr, A.col, A.val, x))
+
+
+def test_spmv_graph_rows(cuda):
+    A = workloads.graph_laplacian(30000, seed=5)
+    _say(f"spmv graph n={A.n} nnz={A.nnz} max_row={np.diff(A.rowptr).max()}")
+    x = workloads.uniform(5, 9, A.n, -1, 1)
+    y = to_dev(cuda, A).spmv(dvec(cuda, x)).cpu().numpy()
+    ref = oracle.spmv(A.rowptr, A.col, A.val, x)
+    short = np.diff(A.rowptr) <= 2048
+    assert np.array_equal(y[short], ref[short])
+    assert np.allclose(y, ref, rtol=1e-13, atol=1e-12)
+
+
+def test_spmv_closed_form_row_sums(cuda):
+    # 27-point, diag 26, off -1: (A*1)_i = 26 - (#in-grid neighbours); interior rows give 0
+    m = 30
+    A = workloads.stencil27(m)
+    y = to_dev(cuda, A).spmv(dvec(cuda, np.ones(A.n))).cpu().numpy().reshape(m, m, m)
+    cnt = np.array([2] + [3] * (m - 2) + [2])
+    nb = cnt[:, None, None] * cnt[None, :, None] * cnt[None, None, :] - 1
+    assert np.array_equal(y, 26.0 - nb)
+
+
+# ----------------------------------------------------------------------------- JPCG
+
+def _solve_both(cuda, A, b, max_iter, tol, records=False):
+    import torch
+    op = to_dev(cuda, A)
+    rec = torch.zeros((max_iter + 2) * 64, dtype=torch.uint8, device=cuda) if records else None
+    x, info = op.solve(dvec(cuda, A.diag), dvec(cuda, b), max_iter, tol, records=rec)
+    ref = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, b, max_iter, tol, records=records)
+    out = {"x": x.cpu().numpy(), "info": info}
+    if records:
+        out["records"] = rec.cpu().numpy().view(oracle.RECORD_DTYPE)
+    return out, ref
+
+
+def test_jpcg_c1_laplace2d_256(cuda):
+    s = workloads.c1_laplace2d(256)
+    _say(f"jpcg C1 n={s.A.n}")
+    g, ref = _solve_both(cuda, s.A, s.b, s.max_iter, s.tol)
+    _say(f"  gpu executed={g['info'].executed} oracle={ref['executed']} res={g['info'].residual:.3e}")
+    assert abs(g["info"].executed - ref["executed"]) <= 1
+    assert rel(g["x"], ref["x"]) <= 1e-10
+    assert rel(g["x"], s.x_true) <= 1e-8
+    assert g["info"].residual < s.tol
+
+
+def test_jpcg_c2_stencil27_small(cuda):
+    s = workloads.c2_stencil27(40)
+    _say(f"jpcg C2-small n={s.A.n}")
+    g, ref = _solve_both(cuda, s.A, s.b, s.max_iter, s.tol)
+    assert abs(g["info"].executed - ref["executed"]) <= 1
+    assert rel(g["x"], ref["x"]) <= 1e-10
+
+
+def test_jpcg_graph_small(cuda):
+    A = workloads.graph_laplacian(20000, seed=3)
+    b = workloads.uniform(3, 4, A.n, -1, 1)
+    _say(f"jpcg graph n={A.n} nnz={A.nnz}")
+    g, ref = _solve_both(cuda, A, b, 20000, 1e-10)
+    _say(f"  gpu executed={g['info'].executed} oracle={ref['executed']}")
+    assert abs(g["info"].executed - ref["executed"]) <= 1
+    assert rel(g["x"], ref["x"]) <= 1e-10
+
+
+def test_jpcg_dense_golden(cuda):
+    z = np.load(os.path.join(GOLDEN, "dense_spd.npz"))
+    for t in range(3):
+        D = z[f"A{t}"]
+        n = D.shape[0]
+        rp = np.arange(0, n * n + 1, n, dtype=np.int32)
+        col = np.tile(np.arange(n, dtype=np.int32), n)
+        A = workloads.CsrMatrix(rp, col, D.ravel().copy(), np.diag(D).copy())
+        g, ref = _solve_both(cuda, A, z[f"b{t}"], 5000, 1e-14)
+        assert abs(g["info"].executed - ref["executed"]) <= 1
+        assert rel(g["x"], z[f"x{t}"]) <= 1e-9
+        assert rel(g["x"], ref["x"]) <= 1e-10
+
+
+def test_jpcg_records_match_oracle(cuda):
+    s = workloads.c1_laplace2d(40)
+    g, ref = _solve_both(cuda, s.A, s.b, 300, 1e-12, records=True)
+    K = g["info"].executed
+    assert abs(K - ref["executed"]) <= 1
+    rg, ro = g["records"], ref["records"]
+    # record 0: the host set-up record
+    assert rg[0]["max_iter"] == 301 and ro[0]["max_iter"] == 301
+    assert rg[0]["vec_size"] == ro[0]["vec_size"] == (s.A.n + 3) // 4 * 4
+    assert np.isclose(rg[0]["res"], ro[0]["res"], rtol=1e-14)
+    assert np.isclose(rg[0]["tols"], ro[0]["tols"], rtol=1e-14)
+    k = min(K, ref["executed"])
+    assert np.array_equal(rg["iter"][1:k + 1], np.arange(k))
+    assert np.array_equal(ro["iter"][1:k + 1], np.arange(k))
+    # early iterations agree closely (later ones drift with rounding)
+    assert np.allclose(rg["alpha"][1:20], ro["alpha"][1:20], rtol=1e-9)
+    assert np.allclose(rg["beta"][1:20], ro["beta"][1:20], rtol=1e-8)
+    assert np.all(rg["max_iter"][K + 1:] == 0)  # unused records stay empty
+
+
+def test_jpcg_iteration_cap_semantics(cuda):
+    # maxIter+1 iterations run, count capped at maxIter, residual of the last iteration
+    s = workloads.c1_laplace2d(64)
+    g, ref = _solve_both(cuda, s.A, s.b, 10, 1e-12)
+    assert g["info"].executed == ref["executed"] == 11
+    assert g["info"].iterations == ref["iter"] == 10
+    assert np.isclose(g["info"].residual, ref["res"], rtol=1e-9)
+
+
+def test_jpcg_diagonal_one_iteration(cuda):
+    n = 1000
+    d = workloads.uniform(4, 0, n, 1.0, 5.0)
+    A = workloads.CsrMatrix(np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), d.copy(), d.copy())
+    b = workloads.uniform(4, 1, n, -1, 1)
+    g, ref = _solve_both(cuda, A, b, 100, 1e-12)
+    assert g["info"].executed == ref["executed"] == 1
+    assert np.allclose(g["x"], b / d, rtol=1e-15)
+
+
+def test_jpcg_zero_rhs(cuda):
+    s = workloads.c1_laplace2d(16)
+    op = to_dev(cuda, s.A)
+    x, info = op.solve(dvec(cuda, s.A.diag), dvec(cuda, np.zeros(s.A.n)), 100, 1e-10)
+    assert info.executed == 0 and info.iterations == 0 and info.residual == 0.0
+    assert not x.cpu().numpy().any()
+
+
+def test_jpcg_deterministic(cuda):
+    s = workloads.c2_stencil27(32)
+    op = to_dev(cuda, s.A)
+    d, b = dvec(cuda, s.A.diag), dvec(cuda, s.b)
+    x1, i1 = op.solve(d, b, s.max_iter, s.tol)
+    x1 = x1.clone()
+    x2, i2 = op.solve(d, b, s.max_iter, s.tol)
+    assert i1.executed == i2.executed
+    assert bool((x1 == x2).all())
+
+
+# ----------------------------------------------------------------------------- pcg.h handle API
+
+def test_pcg_handle_coo_and_reuse_modes(cuda):
+    from hpc_amd.pcg import JPCG, Mode, Status
+    s = workloads.c1_laplace2d(48)
+    A = s.A
+    rows = np.repeat(np.arange(A.n, dtype=np.uint32), np.diff(A.rowptr))
+    # shuffle the COO order: the ingest must not depend on it
+    perm = np.random.default_rng(0).permutation(A.nnz)
+    row, col, val = rows[perm], A.col[perm].astype(np.uint32), A.val[perm]
+    ref = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, s.b, 2000, 1e-12)
+    with JPCG(0) as h:
+        assert h.create_status == Status.SUCCESS, h.last_message
+        out = h.coo_solve(A.n, row, col, val, A.diag, s.b, 2000, 1e-12)
+        assert out.status == Status.SUCCESS, h.last_message
+        assert abs(out.iterations - ref["iter"]) <= 1
+        assert rel(out.x, ref["x"]) <= 1e-10
+        hist = h.history()
+        assert hist.size == out.iterations + 1 and hist[0]["max_iter"] == 2001
+        # KEEP_NZ_LAYOUT: scaled values, NULL indices
+        out2 = h.coo_solve(A.n, None, None, 2.0 * val, 2.0 * A.diag, s.b, 2000, 1e-12, Mode.KEEP_NZ_LAYOUT)
+        assert out2.status == Status.SUCCESS, h.last_message
+        assert rel(out2.x, 0.5 * ref["x"]) <= 1e-10
+        # KEEP_MATRIX: matrix of the previous call, new b
+        out3 = h.coo_solve(A.n, None, None, None, 2.0 * A.diag, 2.0 * s.b, 2000, 1e-12, Mode.KEEP_MATRIX)
+        assert out3.status == Status.SUCCESS, h.last_message
+        assert rel(out3.x, ref["x"]) <= 1e-10
+        # too few iterations: EXECUTION_FAILED, outputs still written
+        out4 = h.coo_solve(A.n, row, col, val, A.diag, s.b, 5, 1e-12)
+        assert out4.status == Status.EXECUTION_FAILED
+        assert out4.iterations == 5 and h.last_message.startswith("Solver ERROR: exit with divergent solution")
+        assert np.abs(out4.x).sum() > 0
+        m = h.metrics()
+        assert m.m_solver > 0 and m.m_matProc > 0
+
+
+def test_pcg_handle_cscsym_fortran(cuda):
+    from hpc_amd.pcg import JPCG, Mode, Status
+    import scipy.sparse as sp
+    s = workloads.c1_laplace2d(40)
+    A = s.A
+    L = sp.tril(s.A.to_scipy()).tocsc()
+    L.sort_indices()
+    ref = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, s.b, 3000, 1e-12)
+    with JPCG() as h:
+        out = h.csc_sym_solve(A.n, L.indices + 1, L.indptr + 1, L.data, A.diag, s.b, 3000, 1e-12,
+                              Mode.DEFAULT | Mode.FORTRAN_INDEX)
+        assert out.status == Status.SUCCESS, h.last_message
+        assert abs(out.iterations - ref["iter"]) <= 1
+        assert rel(out.x, ref["x"]) <= 1e-10
+        # KEEP_NZ_LAYOUT with a mismatched nnz: the reference's partitioner error maps to INTERNAL_ERROR
+        st = h.csc_sym_solve(A.n, L.indices[:-1] + 1, L.indptr + 1, L.data[:-1], A.diag, s.b, 3000, 1e-12,
+                             Mode.KEEP_NZ_LAYOUT | Mode.FORTRAN_INDEX).status
+        assert st == Status.INTERNAL_ERROR
+
+
+def test_example_program_matches_oracle(cuda, tmp_path):
+    """examples/c/example.c sequence through the dlopen loader, against the oracle on the same inputs."""
+    exe = os.path.join(HERE, "c", "example_tridiag")
+    env = dict(os.environ, XJPCG_LIBRARY=os.path.join(os.path.dirname(HERE), "hpc_amd", "libxjpcg.so"))
+    prefix = str(tmp_path / "ex")
+    p = subprocess.run([exe, prefix], env=env, capture_output=True, text=True, timeout=120)
+    _say(p.stdout)
+    assert p.returncode == 0, p.stderr
+    lines = {l.split()[0] + (l.split()[1] if l.startswith("SOLVE") else ""): l.split()
+             for l in p.stdout.splitlines() if l.strip()}
+    n = 65536
+    rows = np.repeat(np.arange(n), 3)[1:-1]
+    A = workloads.CsrMatrix(None, None, None, None)
+    # tridiagonal CSR (diag 2.2, off +1.0)
+    rp = np.zeros(n + 1, np.int32)
+    rp[1:] = np.cumsum(np.r_[2, np.full(n - 2, 3), 2])
+    col = np.concatenate([[0, 1]] + [[i - 1, i, i + 1] for i in range(1, n - 1)] + [[n - 2, n - 1]]).astype(np.int32)
+    val = np.where(col == np.repeat(np.arange(n), np.diff(rp)), 2.2, 1.0)
+    diag = np.full(n, 2.2)
+    del rows, A
+    for k in (1, 2, 3):
+        b = np.fromfile(f"{prefix}.b{k}.bin", dtype=np.float64)
+        x = np.fromfile(f"{prefix}.x{k}.bin", dtype=np.float64)
+        ref = oracle.jpcg(rp, col, val, diag, b, 100, 1e-16)
+        f = lines[f"SOLVE{k}"]
+        st, it = int(f[2]), int(f[3])
+        assert abs(it - ref["iter"]) <= 1, (k, it, ref["iter"])
+        assert st == (0 if ref["res"] <= 1e-16 else 4) or abs(it - ref["iter"]) == 1
+        assert rel(x, ref["x"]) <= 1e-10
