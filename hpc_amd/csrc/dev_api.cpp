@@ -71,6 +71,16 @@ XJPCG_Status_t xjpcg_op_info(const xjpcg_op_t* op, int32_t* n, int64_t* nnz, int
     });
 }
 
+XJPCG_Status_t xjpcg_op_set_path(xjpcg_op_t* op, int path, int* h_last) {
+    return guarded([&] {
+        xj::Operator& o = impl_of(op);
+        if (h_last) *h_last = o.last_path();
+        if (path < 0) return;  // query only
+        if (path > 2) throw xj::invalid("path must be 0, 1 or 2.");
+        o.set_path(path);
+    });
+}
+
 XJPCG_Status_t xjpcg_spmv(xjpcg_op_t* op, const double* d_x, double* d_y, void* stream) {
     return guarded([&] {
         if (d_x == nullptr || d_y == nullptr) throw xj::invalid("Vector is nullptr.");

@@ -147,6 +147,162 @@ __global__ __launch_bounds__(kBlock) void jpcg_persistent(JpcgParams P) {
 }
 
 // --------------------------------------------------------------------------
+// multi-kernel JPCG: three launches per iteration, coherence by kernel boundaries.
+//
+//   cg_spmv      Ap = A p, p.Ap partials               (G1 workgroups, contiguous tile ranges)
+//   cg_update_r  alpha = rz/pAp; r -= alpha Ap; r.r and r.z partials   (G2, grid-stride)
+//   cg_update_xp beta = rz'/rz; x += alpha p; p = beta p + J r; record; stop rule
+//
+// Every workgroup reduces the previous kernel's partials itself in one fixed
+// order, so all workgroups hold bit-identical alpha / beta.  The loop state
+// lives in CgState; each field is written by workgroup 0 of one kernel only
+// and read by the others only in later kernels, so no kernel reads a field
+// that is being written during its own launch.  Once `stop` is set the
+// remaining enqueued launches return immediately.
+
+__global__ __launch_bounds__(kBlock) void cg_setup(CgParams P) {
+    __shared__ double red[kWaves];
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int stride = gridDim.x * kBlock;
+    double dot = 0.0, rz = 0.0;
+    for (int i = L * kBlock + threadIdx.x; i < P.n; i += stride) {
+        const double bi = P.b[i];
+        const double Ji = 1.0 / P.diag[i];
+        const double zi = Ji * bi;
+        P.r[i] = bi;
+        P.J[i] = Ji;
+        P.p[i] = zi;
+        P.x[i] = 0.0;
+        dot += bi * bi;
+        rz += bi * zi;
+    }
+    dot = block_sum(dot, red);
+    rz = block_sum(rz, red);
+    if (threadIdx.x == 0) {
+        P.partB[L] = dot;
+        P.partC[L] = rz;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void cg_setup_finish(CgParams P) {
+    __shared__ double red[kWaves];
+    const double dot = grid_total(P.partB, P.G2, red);
+    const double rz = grid_total(P.partC, P.G2, red);
+    if (threadIdx.x == 0) {
+        CgState* st = P.st;
+        st->dot = dot;
+        st->tols = dot * P.tol * P.tol;
+        st->rr = 0.0;
+        st->rz0 = rz;
+        st->rz[0] = rz;
+        st->rz[1] = 0.0;
+        st->k = 0xFFFFFFFFu;  // the first cg_spmv advances it to 0
+        st->it = 0;
+        st->stop = dot > 0.0 ? 0 : 1;  // b == 0: x = 0, no iteration
+        st->stop_k2 = 0;
+        st->K = 0;
+        if (P.records) {
+            Record rec;
+            rec.id = 0; rec.iter = 0; rec.vec_size = (uint32_t)((P.n + 3) / 4 * 4); rec.max_iter = P.max_rounds;
+            rec.tols = st->tols; rec.res = dot; rec.rz = rz; rec.alpha = 0.0; rec.beta = 0.0; rec.clock = 0;
+            P.records[0] = rec;
+        }
+    }
+}
+
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void cg_spmv(CgParams P) {
+    __shared__ double lds[kTileNnz];
+    __shared__ double red[kWaves];
+    CgState* st = P.st;
+    if (st->stop) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->k = st->k + 1;  // no other workgroup of this launch reads k
+    const int G = gridDim.x;
+    const int L = xcd_remap(blockIdx.x, G);
+    const int t_begin = (int)((int64_t)P.ntiles * L / G);
+    const int t_end = (int)((int64_t)P.ntiles * (L + 1) / G);
+    const VT* __restrict__ val = static_cast<const VT*>(P.val);
+    const double* __restrict__ p = P.p;
+    double pap = 0.0;
+    for (int t = t_begin; t < t_end; ++t) {
+        double y;
+        int row;
+        tile_spmv<VT>(P.tile_row[t], P.tile_row[t + 1], P.rowptr, P.col, val, P.dsplit, p, lds, red, &y, &row);
+        if (row >= 0) {
+            P.Ap[row] = y;
+            pap += p[row] * y;
+        }
+    }
+    pap = block_sum(pap, red);
+    if (threadIdx.x == 0) P.partA[L] = pap;
+}
+
+__global__ __launch_bounds__(kBlock) void cg_update_r(CgParams P) {
+    __shared__ double red[kWaves];
+    CgState* st = P.st;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int stop = st->stop;
+    if (L == 0 && threadIdx.x == 0) st->stop_k2 = stop;
+    if (stop) return;
+    const uint32_t k = st->k;
+    const double alpha = st->rz[k & 1] / grid_total(P.partA, P.G1, red);
+    const int stride = gridDim.x * kBlock;
+    double rr = 0.0, rz = 0.0;
+    for (int i = L * kBlock + threadIdx.x; i < P.n; i += stride) {
+        const double ri = -alpha * P.Ap[i] + P.r[i];
+        P.r[i] = ri;
+        const double zi = P.J[i] * ri;
+        rr += ri * ri;
+        rz += ri * zi;
+    }
+    rr = block_sum(rr, red);
+    rz = block_sum(rz, red);
+    if (threadIdx.x == 0) {
+        P.partB[L] = rr;
+        P.partC[L] = rz;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void cg_update_xp(CgParams P) {
+    __shared__ double red[kWaves];
+    CgState* st = P.st;
+    if (st->stop_k2) return;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t k = st->k;
+    const double rz = st->rz[k & 1];
+    const double alpha = rz / grid_total(P.partA, P.G1, red);
+    const double rr = grid_total(P.partB, P.G2, red);
+    const double rzn = grid_total(P.partC, P.G2, red);
+    const double beta = rzn / rz;
+    const int stride = gridDim.x * kBlock;
+    for (int i = L * kBlock + threadIdx.x; i < P.n; i += stride) {
+        const double pi = P.p[i];
+        P.x[i] = alpha * pi + P.x[i];
+        P.p[i] = beta * pi + P.J[i] * P.r[i];
+    }
+    if (L == 0 && threadIdx.x == 0) {
+        st->rz[(k + 1) & 1] = rzn;
+        st->rr = rr;
+        st->K = (int)(k + 1);
+        uint32_t it = st->it;
+        if (P.records) {
+            Record rec;
+            rec.id = 0; rec.iter = it; rec.vec_size = (uint32_t)((P.n + 3) / 4 * 4); rec.max_iter = P.max_rounds;
+            rec.tols = st->tols; rec.res = rr; rec.rz = rzn; rec.alpha = alpha; rec.beta = beta;
+            rec.clock = wall_clock64();
+            P.records[k + 1] = rec;
+        }
+        bool more = false;
+        if (!(rr < st->tols)) {
+            ++it;
+            more = it < P.max_rounds;
+        }
+        st->it = it;
+        if (!more) st->stop = 1;
+    }
+}
+
+// --------------------------------------------------------------------------
 // standalone kernels
 
 template <typename VT>
@@ -251,6 +407,22 @@ hipError_t launch_jpcg(const JpcgParams& P0, bool f32, int G, hipStream_t s) {
     if (f32)
         return hipLaunchCooperativeKernel((const void*)jpcg_persistent<float>, dim3(G), dim3(kBlock), args, 0, s);
     return hipLaunchCooperativeKernel((const void*)jpcg_persistent<double>, dim3(G), dim3(kBlock), args, 0, s);
+}
+
+hipError_t launch_cg_setup(const CgParams& P, hipStream_t s) {
+    cg_setup<<<P.G2, kBlock, 0, s>>>(P);
+    cg_setup_finish<<<1, kBlock, 0, s>>>(P);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg_iteration(const CgParams& P, bool f32, hipStream_t s) {
+    if (f32)
+        cg_spmv<float><<<P.G1, kBlock, 0, s>>>(P);
+    else
+        cg_spmv<double><<<P.G1, kBlock, 0, s>>>(P);
+    cg_update_r<<<P.G2, kBlock, 0, s>>>(P);
+    cg_update_xp<<<P.G2, kBlock, 0, s>>>(P);
+    return hipGetLastError();
 }
 
 hipError_t launch_spmv(const CsrDev& A, const double* x, double* y, const double* b, hipStream_t s) {

@@ -66,22 +66,30 @@ __device__ __forceinline__ double grid_total(const double* part, int G, double* 
 
 // Grid-wide barrier for cooperatively launched kernels (all workgroups
 // co-resident).  `bar` is a zeroed 64-bit arrival counter; `target` is the
-// caller's running count of expected arrivals.  A wait longer than
-// kBarrierTimeout ticks of the 100 MHz wall clock sets *abortf and every
-// workgroup leaves the barrier; returns false once the abort flag is set.
+// caller's running count of expected arrivals.
+//
+// On a multi-XCD part an agent-scope release writes back the XCD's L2 and an
+// acquire invalidates it, so only lane 0 of each workgroup fences, once on
+// each side, and the spin uses relaxed loads (an acquire load per poll would
+// invalidate L2 under the workgroups still computing).  The __syncthreads
+// before the release orders the other wavefronts' stores ahead of it.
+//
+// A wait longer than kBarrierTimeout ticks of the 100 MHz wall clock sets
+// *abortf and every workgroup leaves the barrier; returns false once the
+// abort flag is set.
 constexpr uint64_t kBarrierTimeout = 500000000ull;  // 5 s
 
 __device__ __forceinline__ bool grid_barrier(unsigned long long* bar, unsigned long long& target, int G,
                                              int* abortf, int* sflag) {
-    __threadfence();
     __syncthreads();
     target += (unsigned long long)G;
     if (threadIdx.x == 0) {
-        atomicAdd(bar, 1ull);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(bar, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const uint64_t t0 = wall_clock64();
         int ab = 0;
-        while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            __builtin_amdgcn_s_sleep(1);
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(2);
             if (__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { ab = 1; break; }
             if (wall_clock64() - t0 > kBarrierTimeout) {
                 __hip_atomic_store(abortf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -89,10 +97,10 @@ __device__ __forceinline__ bool grid_barrier(unsigned long long* bar, unsigned l
                 break;
             }
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         *sflag = ab;
     }
     __syncthreads();
-    __threadfence();
     return *sflag == 0;
 }
 

@@ -45,6 +45,40 @@ struct JpcgParams {
     uint32_t max_rounds = 0;             // maxIter + 1, as the reference host programs it
 };
 
+// Device state of the multi-kernel CG loop (one per operator).
+struct CgState {
+    double dot, tols, rr, rz0;
+    double rz[2];      // rz of iteration k lives in rz[k & 1]
+    uint32_t k;        // index of the running iteration; advanced by workgroup 0 of the SpMV kernel
+    uint32_t it;       // the reference Task's iteration counter (tasks.hpp:30-47)
+    int stop;          // set by workgroup 0 of the update kernel when the loop ends
+    int stop_k2;       // snapshot of `stop` taken by the residual kernel
+    int K;             // iterations executed
+    int pad;
+};
+
+// Arguments of the multi-kernel loop: SpMV (G1 workgroups, tile loop), then
+// two vector kernels (G2 workgroups, grid-stride).
+struct CgParams {
+    int n = 0, ntiles = 0, G1 = 0, G2 = 0;
+    const int* rowptr = nullptr;
+    const int* col = nullptr;
+    const void* val = nullptr;
+    const double* dsplit = nullptr;
+    const int* tile_row = nullptr;
+    const double* b = nullptr;
+    const double* diag = nullptr;
+    double *x = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr, *J = nullptr;
+    double *partA = nullptr, *partB = nullptr, *partC = nullptr;  // G1, G2, G2 doubles
+    CgState* st = nullptr;
+    Record* records = nullptr;
+    double tol = 0.0;
+    uint32_t max_rounds = 0;
+};
+
+hipError_t launch_cg_setup(const CgParams& P, hipStream_t s);
+hipError_t launch_cg_iteration(const CgParams& P, bool f32, hipStream_t s);
+
 hipError_t persistent_grid(bool f32, int* G);
 hipError_t launch_jpcg(const JpcgParams& P, bool f32, int G, hipStream_t s);
 hipError_t launch_spmv(const CsrDev& A, const double* x, double* y, const double* b, hipStream_t s);

@@ -1,6 +1,7 @@
 // C++ host engine core (see operator.hpp).
 #include "operator.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -62,7 +63,19 @@ Operator::Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t*
     A_.tile_row = tiles_.ptr;
     A_.ntiles = (int)t.size() - 1;
     hip_check(persistent_grid(val_f32, &G_), "persistent kernel occupancy");
-    hip_check(hipHostMalloc((void**)&host_result_, 8 * sizeof(double), hipHostMallocDefault), "pinned result");
+    G1_ = std::min(A_.ntiles, 2048);
+    if (G1_ < 1) G1_ = 1;
+    G2_ = std::min((n + 1023) / 1024, 2048);
+    if (G2_ < 1) G2_ = 1;
+    partA_.ensure(G1_);
+    partB_.ensure(G2_);
+    partC_.ensure(G2_);
+    state_.ensure(1);
+    hip_check(hipEventCreateWithFlags(&ev_[0], hipEventDisableTiming), "event");
+    hip_check(hipEventCreateWithFlags(&ev_[1], hipEventDisableTiming), "event");
+    hip_check(hipHostMalloc((void**)&host_flags_, 2 * sizeof(int), hipHostMallocDefault), "pinned flags");
+    hip_check(hipHostMalloc((void**)&host_result_, 16 * sizeof(double), hipHostMallocDefault), "pinned result");
+    static_assert(sizeof(CgState) <= 16 * sizeof(double), "state fits the pinned result buffer");
     // the vectors are only needed by solve(); allocate them now so a solve never allocates
     r_.ensure(n);
     p_.ensure(n);
@@ -77,6 +90,9 @@ Operator::Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t*
 
 Operator::~Operator() {
     if (host_result_) (void)hipHostFree(host_result_);
+    if (host_flags_) (void)hipHostFree(host_flags_);
+    for (hipEvent_t e : ev_)
+        if (e) (void)hipEventDestroy(e);
 }
 
 void Operator::spmv(const double* x, double* y, hipStream_t s) const {
@@ -102,6 +118,86 @@ void Operator::sddmm(const double* lam, const double* x, double* g, hipStream_t 
 SolveResult Operator::solve(const double* diag, const double* b, double* x, uint32_t max_iter, double tol,
                             void* records, hipStream_t s) {
     if (diag == nullptr || b == nullptr || x == nullptr) throw invalid("Vector is nullptr.");
+    if (max_iter == 0xFFFFFFFFu) throw invalid("maxIter too large.");
+    int p = path_;
+    if (p == kAuto) p = kMultiKernel;
+    if (!coop_ok_) p = kMultiKernel;
+    last_path_ = p;
+    if (p == kPersistent) return solve_persistent(diag, b, x, max_iter, tol, records, s);
+    return solve_multi(diag, b, x, max_iter, tol, records, s);
+}
+
+static SolveResult finish_result(int64_t K, double rr, double dot, uint32_t max_iter) {
+    SolveResult R;
+    R.executed = K;
+    R.rr = rr;
+    R.dot = dot;
+    // result scan of pcgImp.hpp:171-180: count capped at maxIter
+    R.iter = R.executed < (int64_t)max_iter ? (uint32_t)R.executed : max_iter;
+    R.res = R.dot > 0.0 ? std::sqrt(R.rr / R.dot) : 0.0;
+    return R;
+}
+
+SolveResult Operator::solve_multi(const double* diag, const double* b, double* x, uint32_t max_iter, double tol,
+                                  void* records, hipStream_t s) {
+    CgParams P;
+    P.n = A_.n;
+    P.ntiles = A_.ntiles;
+    P.G1 = G1_;
+    P.G2 = G2_;
+    P.rowptr = A_.rowptr;
+    P.col = A_.col;
+    P.val = A_.val;
+    P.dsplit = A_.dsplit;
+    P.tile_row = A_.tile_row;
+    P.b = b;
+    P.diag = diag;
+    P.x = x;
+    P.r = r_.ptr;
+    P.p = p_.ptr;
+    P.Ap = Ap_.ptr;
+    P.J = J_.ptr;
+    P.partA = partA_.ptr;
+    P.partB = partB_.ptr;
+    P.partC = partC_.ptr;
+    P.st = state_.ptr;
+    P.records = static_cast<Record*>(records);
+    P.tol = tol;
+    P.max_rounds = max_iter + 1u;  // pcgImp.hpp:158 programs maxIter+1 rounds
+    if (records) hip_check(hipMemsetAsync(records, 0, (size_t)(max_iter + 2) * 64, s), "clear records");
+    hip_check(launch_cg_setup(P, s), "CG set-up launch");
+    // Enqueue iterations in growing chunks; after each chunk the stop flag is
+    // copied to pinned memory.  The host checks chunk c-1's flag while chunk c
+    // is queued, so the GPU never idles and at most one chunk of launches
+    // returns immediately after convergence.
+    uint64_t enq = 0;
+    uint32_t chunk = 4;
+    int slot = 0;
+    bool pending = false;
+    while (enq < P.max_rounds) {
+        const uint32_t c = (uint32_t)std::min<uint64_t>(chunk, P.max_rounds - enq);
+        for (uint32_t j = 0; j < c; ++j) hip_check(launch_cg_iteration(P, A_.val_f32, s), "CG iteration launch");
+        enq += c;
+        hip_check(hipMemcpyAsync(&host_flags_[slot], &state_.ptr->stop, sizeof(int), hipMemcpyDeviceToHost, s),
+                  "read stop flag");
+        hip_check(hipEventRecord(ev_[slot], s), "event record");
+        if (pending) {
+            hip_check(hipEventSynchronize(ev_[slot ^ 1]), "event sync");
+            if (host_flags_[slot ^ 1]) break;
+        }
+        pending = true;
+        slot ^= 1;
+        chunk = std::min<uint32_t>(chunk * 2, 64);
+    }
+    CgState hs;
+    hip_check(hipMemcpyAsync(host_result_, state_.ptr, sizeof(CgState), hipMemcpyDeviceToHost, s), "read state");
+    hip_check(hipStreamSynchronize(s), "JPCG solve");
+    std::memcpy(&hs, host_result_, sizeof(CgState));
+    return finish_result(hs.K, hs.rr, hs.dot, max_iter);
+}
+
+SolveResult Operator::solve_persistent(const double* diag, const double* b, double* x, uint32_t max_iter,
+                                       double tol, void* records, hipStream_t s) {
     JpcgParams P;
     P.n = A_.n;
     P.ntiles = A_.ntiles;
@@ -128,19 +224,28 @@ SolveResult Operator::solve(const double* diag, const double* b, double* x, uint
     hip_check(hipMemsetAsync(bar_.ptr, 0, sizeof(unsigned long long), s), "reset barrier");
     hip_check(hipMemsetAsync(abort_.ptr, 0, sizeof(int), s), "reset abort flag");
     if (records) hip_check(hipMemsetAsync(records, 0, (size_t)(max_iter + 2) * 64, s), "clear records");
-    hip_check(launch_jpcg(P, A_.val_f32, G_, s), "persistent JPCG launch");
+    hipError_t e = launch_jpcg(P, A_.val_f32, G_, s);
+    if (e != hipSuccess) {
+        // Some runtimes refuse the cooperative launch (grid above what they will
+        // make co-resident, or no cooperative support on the queue).  Clear the
+        // error, try half the grid once, then use the multi-kernel loop for good.
+        (void)hipGetLastError();
+        if (G_ > 1) {
+            G_ /= 2;
+            e = launch_jpcg(P, A_.val_f32, G_, s);
+        }
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            coop_ok_ = false;
+            last_path_ = kMultiKernel;
+            return solve_multi(diag, b, x, max_iter, tol, records, s);
+        }
+    }
     hip_check(hipMemcpyAsync(host_result_, result_.ptr, 5 * sizeof(double), hipMemcpyDeviceToHost, s),
               "read solve result");
     hip_check(hipStreamSynchronize(s), "JPCG solve");
     if (host_result_[4] != 0.0) throw internal("grid barrier timed out; solve aborted.");
-    SolveResult R;
-    R.executed = (int64_t)host_result_[0];
-    R.rr = host_result_[1];
-    R.dot = host_result_[2];
-    // result scan of pcgImp.hpp:171-180: count capped at maxIter
-    R.iter = R.executed < (int64_t)max_iter ? (uint32_t)R.executed : max_iter;
-    R.res = R.dot > 0.0 ? std::sqrt(R.rr / R.dot) : 0.0;
-    return R;
+    return finish_result((int64_t)host_result_[0], host_result_[1], host_result_[2], max_iter);
 }
 
 }  // namespace xj

@@ -92,8 +92,28 @@ class Operator {
     SolveResult solve(const double* diag, const double* b, double* x, uint32_t max_iter, double tol, void* records,
                       hipStream_t s);
 
+    // Which loop solve() runs: kPersistent is one cooperative launch with grid
+    // barriers, kMultiKernel is three launches per iteration; kAuto picks the
+    // faster measured one (the multi-kernel loop at every size measured so far).
+    enum Path { kAuto = 0, kPersistent = 1, kMultiKernel = 2 };
+    void set_path(int p) { path_ = p; }
+    int path() const { return path_; }
+    int last_path() const { return last_path_; }
+
   private:
+    SolveResult solve_persistent(const double* diag, const double* b, double* x, uint32_t max_iter, double tol,
+                                 void* records, hipStream_t s);
+    SolveResult solve_multi(const double* diag, const double* b, double* x, uint32_t max_iter, double tol,
+                            void* records, hipStream_t s);
+
     CsrDev A_;
+    int path_ = kAuto, last_path_ = 0;
+    bool coop_ok_ = true;  // false once the runtime refused the cooperative launch
+    int G1_ = 0, G2_ = 0;
+    DevBuf<double> partA_, partB_, partC_;
+    DevBuf<CgState> state_;
+    hipEvent_t ev_[2] = {nullptr, nullptr};
+    int* host_flags_ = nullptr;  // pinned
     DevBuf<int32_t> tiles_;
     int G_ = 0;
     DevBuf<double> r_, p_, Ap_, J_, part_, result_;

@@ -90,6 +90,18 @@ class CsrOperator:
                                       ctypes.byref(g)))
         return {"n": n.value, "nnz": nnz.value, "ntiles": nt.value, "grid": g.value}
 
+    PATH_AUTO, PATH_PERSISTENT, PATH_MULTI = 0, 1, 2
+
+    def set_path(self, path: int):
+        """Solver loop: 0 auto, 1 persistent cooperative kernel, 2 three launches per iteration."""
+        check_dev(lib().xjpcg_op_set_path(self._h, int(path), None))
+
+    @property
+    def last_path(self) -> int:
+        last = ctypes.c_int()
+        check_dev(lib().xjpcg_op_set_path(self._h, -1, ctypes.byref(last)))
+        return last.value
+
     def spmv(self, x: torch.Tensor, y: torch.Tensor | None = None) -> torch.Tensor:
         _dev_check(x, torch.float64, "x", self.n)
         if y is None:

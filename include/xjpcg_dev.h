@@ -44,6 +44,14 @@ XJPCG_Status_t xjpcg_op_destroy(xjpcg_op_t* op);
 /* n, nnz, number of row tiles, grid of the persistent kernel */
 XJPCG_Status_t xjpcg_op_info(const xjpcg_op_t* op, int32_t* n, int64_t* nnz, int32_t* ntiles, int32_t* grid);
 
+/* Loop used by xjpcg_solve on this operator: 0 = automatic (the faster
+ * measured loop), 1 = one persistent cooperative launch per solve with grid
+ * barriers, 2 = three launches per iteration (SpMV with the p.Ap partials,
+ * residual update, x / p update) with the loop state on the device.  A
+ * runtime that refuses the cooperative launch gets loop 2.  *h_last (may be
+ * NULL) receives the loop the last solve used; path < 0 only queries. */
+XJPCG_Status_t xjpcg_op_set_path(xjpcg_op_t* op, int path, int* h_last);
+
 /* d_y = A d_x  (the reference's SpMV compute units, L2/sparse/src/hw/fp64) */
 XJPCG_Status_t xjpcg_spmv(xjpcg_op_t* op, const double* d_x, double* d_y, void* stream);
 

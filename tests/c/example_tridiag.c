@@ -95,6 +95,7 @@ int main(int argc, char** argv) {
     make_b(n, b);
     s = xJPCG_cooSolver(h, n, nnz, row, col, val, diag, b, x, max_iter, tol, &it, &res, XJPCG_MODE_DEFAULT);
     printf("SOLVE 1 %d %u %.17g\n", (int)s, it, res);
+    printf("MSG 1 %s\n", xJPCG_getLastMessage(h));
     dump(prefix, "b", 1, b, n);
     dump(prefix, "x", 1, x, n);
 
@@ -102,6 +103,7 @@ int main(int argc, char** argv) {
     make_b(n, b);
     s = xJPCG_cooSolver(h, n, nnz, NULL, NULL, val, diag, b, x, max_iter, tol, &it, &res, XJPCG_MODE_KEEP_NZ_LAYOUT);
     printf("SOLVE 2 %d %u %.17g\n", (int)s, it, res);
+    printf("MSG 2 %s\n", xJPCG_getLastMessage(h));
     dump(prefix, "b", 2, b, n);
     dump(prefix, "x", 2, x, n);
 
@@ -111,6 +113,7 @@ int main(int argc, char** argv) {
     s = xJPCG_cscSymSolver(h, n, half, row_idx, col_ptr, hval, diag, b, x, max_iter, tol, &it64, &res,
                            (XJPCG_Mode_t)(XJPCG_MODE_DEFAULT | XJPCG_MODE_FORTRAN_INDEX));
     printf("SOLVE 3 %d %lld %.17g\n", (int)s, (long long)it64, res);
+    printf("MSG 3 %s\n", xJPCG_getLastMessage(h));
     dump(prefix, "b", 3, b, n);
     dump(prefix, "x", 3, x, n);
 

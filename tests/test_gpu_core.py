@@ -90,9 +90,13 @@ def test_spmv_closed_form_row_sums(cuda):
 
 # ----------------------------------------------------------------------------- JPCG
 
-def _solve_both(cuda, A, b, max_iter, tol, records=False):
+PATHS = pytest.mark.parametrize("path", [1, 2], ids=["persistent", "multikernel"])
+
+
+def _solve_both(cuda, A, b, max_iter, tol, records=False, path=0):
     import torch
     op = to_dev(cuda, A)
+    op.set_path(path)
     rec = torch.zeros((max_iter + 2) * 64, dtype=torch.uint8, device=cuda) if records else None
     x, info = op.solve(dvec(cuda, A.diag), dvec(cuda, b), max_iter, tol, records=rec)
     ref = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, b, max_iter, tol, records=records)
@@ -102,10 +106,11 @@ def _solve_both(cuda, A, b, max_iter, tol, records=False):
     return out, ref
 
 
-def test_jpcg_c1_laplace2d_256(cuda):
+@PATHS
+def test_jpcg_c1_laplace2d_256(cuda, path):
     s = workloads.c1_laplace2d(256)
     _say(f"jpcg C1 n={s.A.n}")
-    g, ref = _solve_both(cuda, s.A, s.b, s.max_iter, s.tol)
+    g, ref = _solve_both(cuda, s.A, s.b, s.max_iter, s.tol, path=path)
     _say(f"  gpu executed={g['info'].executed} oracle={ref['executed']} res={g['info'].residual:.3e}")
     assert abs(g["info"].executed - ref["executed"]) <= 1
     assert rel(g["x"], ref["x"]) <= 1e-10
@@ -113,25 +118,28 @@ def test_jpcg_c1_laplace2d_256(cuda):
     assert g["info"].residual < s.tol
 
 
-def test_jpcg_c2_stencil27_small(cuda):
+@PATHS
+def test_jpcg_c2_stencil27_small(cuda, path):
     s = workloads.c2_stencil27(40)
     _say(f"jpcg C2-small n={s.A.n}")
-    g, ref = _solve_both(cuda, s.A, s.b, s.max_iter, s.tol)
+    g, ref = _solve_both(cuda, s.A, s.b, s.max_iter, s.tol, path=path)
     assert abs(g["info"].executed - ref["executed"]) <= 1
     assert rel(g["x"], ref["x"]) <= 1e-10
 
 
-def test_jpcg_graph_small(cuda):
+@PATHS
+def test_jpcg_graph_small(cuda, path):
     A = workloads.graph_laplacian(20000, seed=3)
     b = workloads.uniform(3, 4, A.n, -1, 1)
     _say(f"jpcg graph n={A.n} nnz={A.nnz}")
-    g, ref = _solve_both(cuda, A, b, 20000, 1e-10)
+    g, ref = _solve_both(cuda, A, b, 20000, 1e-10, path=path)
     _say(f"  gpu executed={g['info'].executed} oracle={ref['executed']}")
     assert abs(g["info"].executed - ref["executed"]) <= 1
     assert rel(g["x"], ref["x"]) <= 1e-10
 
 
-def test_jpcg_dense_golden(cuda):
+@PATHS
+def test_jpcg_dense_golden(cuda, path):
     z = np.load(os.path.join(GOLDEN, "dense_spd.npz"))
     for t in range(3):
         D = z[f"A{t}"]
@@ -139,15 +147,16 @@ def test_jpcg_dense_golden(cuda):
         rp = np.arange(0, n * n + 1, n, dtype=np.int32)
         col = np.tile(np.arange(n, dtype=np.int32), n)
         A = workloads.CsrMatrix(rp, col, D.ravel().copy(), np.diag(D).copy())
-        g, ref = _solve_both(cuda, A, z[f"b{t}"], 5000, 1e-14)
+        g, ref = _solve_both(cuda, A, z[f"b{t}"], 5000, 1e-14, path=path)
         assert abs(g["info"].executed - ref["executed"]) <= 1
         assert rel(g["x"], z[f"x{t}"]) <= 1e-9
         assert rel(g["x"], ref["x"]) <= 1e-10
 
 
-def test_jpcg_records_match_oracle(cuda):
+@PATHS
+def test_jpcg_records_match_oracle(cuda, path):
     s = workloads.c1_laplace2d(40)
-    g, ref = _solve_both(cuda, s.A, s.b, 300, 1e-12, records=True)
+    g, ref = _solve_both(cuda, s.A, s.b, 300, 1e-12, records=True, path=path)
     K = g["info"].executed
     assert abs(K - ref["executed"]) <= 1
     rg, ro = g["records"], ref["records"]
@@ -165,36 +174,42 @@ def test_jpcg_records_match_oracle(cuda):
     assert np.all(rg["max_iter"][K + 1:] == 0)  # unused records stay empty
 
 
-def test_jpcg_iteration_cap_semantics(cuda):
+@PATHS
+def test_jpcg_iteration_cap_semantics(cuda, path):
     # maxIter+1 iterations run, count capped at maxIter, residual of the last iteration
     s = workloads.c1_laplace2d(64)
-    g, ref = _solve_both(cuda, s.A, s.b, 10, 1e-12)
+    g, ref = _solve_both(cuda, s.A, s.b, 10, 1e-12, path=path)
     assert g["info"].executed == ref["executed"] == 11
     assert g["info"].iterations == ref["iter"] == 10
     assert np.isclose(g["info"].residual, ref["res"], rtol=1e-9)
 
 
-def test_jpcg_diagonal_one_iteration(cuda):
+@PATHS
+def test_jpcg_diagonal_one_iteration(cuda, path):
     n = 1000
     d = workloads.uniform(4, 0, n, 1.0, 5.0)
     A = workloads.CsrMatrix(np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), d.copy(), d.copy())
     b = workloads.uniform(4, 1, n, -1, 1)
-    g, ref = _solve_both(cuda, A, b, 100, 1e-12)
+    g, ref = _solve_both(cuda, A, b, 100, 1e-12, path=path)
     assert g["info"].executed == ref["executed"] == 1
     assert np.allclose(g["x"], b / d, rtol=1e-15)
 
 
-def test_jpcg_zero_rhs(cuda):
+@PATHS
+def test_jpcg_zero_rhs(cuda, path):
     s = workloads.c1_laplace2d(16)
     op = to_dev(cuda, s.A)
+    op.set_path(path)
     x, info = op.solve(dvec(cuda, s.A.diag), dvec(cuda, np.zeros(s.A.n)), 100, 1e-10)
     assert info.executed == 0 and info.iterations == 0 and info.residual == 0.0
     assert not x.cpu().numpy().any()
 
 
-def test_jpcg_deterministic(cuda):
+@PATHS
+def test_jpcg_deterministic(cuda, path):
     s = workloads.c2_stencil27(32)
     op = to_dev(cuda, s.A)
+    op.set_path(path)
     d, b = dvec(cuda, s.A.diag), dvec(cuda, s.b)
     x1, i1 = op.solve(d, b, s.max_iter, s.tol)
     x1 = x1.clone()
