@@ -35,9 +35,7 @@ __global__ __launch_bounds__(kBlock) void jpcg_persistent(JpcgParams P) {
     double* __restrict__ J = P.J;
     double* part = P.part;
     unsigned long long target = 0;
-    // tiles of this workgroup: a contiguous range, so each XCD sweeps one band of rows
-    const int t_begin = (int)((int64_t)P.ntiles * L / G);
-    const int t_end = (int)((int64_t)P.ntiles * (L + 1) / G);
+    const TileSweep sw = tile_sweep(L, G, P.ntiles);
 
     // set-up (reference cgVector.hpp:101-114): r0 = b, J = 1/diag, z0 = J r0, p0 = z0, x0 = 0
     double dot = 0.0, rz = 0.0;
@@ -80,7 +78,7 @@ __global__ __launch_bounds__(kBlock) void jpcg_persistent(JpcgParams P) {
         for (;;) {
             // A: Ap = A p and the p.Ap partial (reference krnl_storeApk.cpp:63-76)
             double pap = 0.0;
-            for (int t = t_begin; t < t_end; ++t) {
+            for (int t = sw.first; t < sw.end; t += sw.step) {
                 double y;
                 int row;
                 tile_spmv<VT>(P.tile_row[t], P.tile_row[t + 1], P.rowptr, P.col, val, P.dsplit, p, lds, red, &y,
@@ -219,12 +217,11 @@ __global__ __launch_bounds__(kBlock) void cg_spmv(CgParams P) {
     if (blockIdx.x == 0 && threadIdx.x == 0) st->k = st->k + 1;  // no other workgroup of this launch reads k
     const int G = gridDim.x;
     const int L = xcd_remap(blockIdx.x, G);
-    const int t_begin = (int)((int64_t)P.ntiles * L / G);
-    const int t_end = (int)((int64_t)P.ntiles * (L + 1) / G);
+    const TileSweep sw = tile_sweep(L, G, P.ntiles);
     const VT* __restrict__ val = static_cast<const VT*>(P.val);
     const double* __restrict__ p = P.p;
     double pap = 0.0;
-    for (int t = t_begin; t < t_end; ++t) {
+    for (int t = sw.first; t < sw.end; t += sw.step) {
         double y;
         int row;
         tile_spmv<VT>(P.tile_row[t], P.tile_row[t + 1], P.rowptr, P.col, val, P.dsplit, p, lds, red, &y, &row);

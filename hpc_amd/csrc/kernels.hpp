@@ -141,34 +141,80 @@ __device__ __forceinline__ void tile_spmv(int r0, int r1, const int* __restrict_
         *row = (tid == 0) ? r0 : -1;
         return;
     }
-    int c[kPerThread];
-    VT v[kPerThread];
+    const int cnt = k1 - k0;
+    if (cnt > 0) {
+        // Branch-free streaming: lanes past the tile's end re-read entry k0
+        // (a valid address) and their products are dropped, so all loads of a
+        // lane issue back to back and the gathers overlap instead of waiting
+        // one by one.
+        int c[kPerThread];
+        VT v[kPerThread];
 #pragma unroll
-    for (int i = 0; i < kPerThread; ++i) {
-        const int k = k0 + tid + i * kBlock;
-        c[i] = 0;
-        v[i] = VT(0);
-        if (k < k1) {
+        for (int i = 0; i < kPerThread; ++i) {
+            const int e = tid + i * kBlock;
+            const int k = k0 + (e < cnt ? e : 0);
             c[i] = ld_stream(col + k);
             v[i] = ld_stream(val + k);
         }
-    }
+        double g[kPerThread];
 #pragma unroll
-    for (int i = 0; i < kPerThread; ++i) {
-        const int k = k0 + tid + i * kBlock;
-        if (k < k1) lds[k - k0] = (double)v[i] * x[c[i]];
+        for (int i = 0; i < kPerThread; ++i) g[i] = x[c[i]];
+#pragma unroll
+        for (int i = 0; i < kPerThread; ++i) {
+            const int e = tid + i * kBlock;
+            if (e < cnt) lds[e] = (double)v[i] * g[i];
+        }
     }
     __syncthreads();
     if (tid < r1 - r0) {
         const int a = rowptr[r0 + tid] - k0, b = rowptr[r0 + tid + 1] - k0;
         double s = dsplit ? dsplit[r0 + tid] * x[r0 + tid] : 0.0;
-        for (int j = a; j < b; ++j) s += lds[j];
+        // left-to-right sum; the LDS reads of a group of four issue together
+        int j = a;
+        for (; j + 4 <= b; j += 4) {
+            const double q0 = lds[j], q1 = lds[j + 1], q2 = lds[j + 2], q3 = lds[j + 3];
+            s += q0;
+            s += q1;
+            s += q2;
+            s += q3;
+        }
+        for (; j < b; ++j) s += lds[j];
         *y = s;
         *row = r0 + tid;
     } else {
         *row = -1;
     }
     __syncthreads();
+}
+
+// Tiles of logical workgroup L in a grid of G for a tile loop: the logical
+// workgroups of one XCD (a contiguous range under xcd_remap) own one
+// contiguous eighth of the tiles and deal them out round-robin, so the XCD's
+// resident workgroups sweep its band of rows together and the gathered
+// x-vector window stays in that XCD's L2.  Tiles of L are
+// first, first + step, ... below end.
+struct TileSweep {
+    int first, step, end;
+};
+__device__ __forceinline__ TileSweep tile_sweep(int L, int G, int ntiles) {
+    const int groups = G < 8 ? G : 8;
+    const int q = G / groups, r = G % groups;  // the first r groups hold q+1 workgroups
+    int grp, l, size;
+    if (L < r * (q + 1)) {
+        grp = L / (q + 1);
+        l = L % (q + 1);
+        size = q + 1;
+    } else {
+        grp = r + (L - r * (q + 1)) / q;
+        l = (L - r * (q + 1)) % q;
+        size = q;
+    }
+    TileSweep s;
+    const int gbeg = (int)((int64_t)ntiles * grp / groups);
+    s.end = (int)((int64_t)ntiles * (grp + 1) / groups);
+    s.first = gbeg + l;
+    s.step = size;
+    return s;
 }
 
 }  // namespace xj

@@ -211,9 +211,9 @@ class PcgHandle {
         if (n == 0) throw invalid("Wrong dimension size.");
         if (nnz == 0) throw invalid("Wrong non-zero element size.");
         if (row == nullptr || col == nullptr || data == nullptr) throw invalid("Matrix is nullptr.");
-        require_device();
         Pattern P;
         pattern_from_coo(P, n, nnz, row, col, base);
+        require_device();
         upload_pattern(std::move(P), data);
     }
 
@@ -234,9 +234,9 @@ class PcgHandle {
         if (n == 0) throw invalid("Wrong dimension size.");
         if (nnz_half == 0) throw invalid("Wrong non-zero element size.");
         if (row_idx == nullptr || col_ptr == nullptr || data == nullptr) throw invalid("Matrix is nullptr.");
-        require_device();
         Pattern P;
         pattern_from_cscsym(P, n, nnz_half, row_idx, col_ptr, base);
+        require_device();
         upload_pattern(std::move(P), data);
     }
 

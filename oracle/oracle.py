@@ -57,7 +57,15 @@ def ref_lib():
                 return None
         if not os.path.exists(REF_LIB):
             return None
-        _R = ctypes.CDLL(REF_LIB)
+        R = ctypes.CDLL(REF_LIB)
+        v, u32, i, d = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_double
+        R.ref_cg_init.argtypes = [u32, v, v, ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(u32), v, v, v]
+        R.ref_record0.argtypes = [u32, u32, d, d, d, v]
+        R.ref_record_encode.argtypes = [u32] * 4 + [d] * 5 + [ctypes.c_uint64, v]
+        R.ref_record_decode.argtypes = [v, v, v, v]
+        R.ref_load_coo.argtypes = [u32, u32, v, v, i, v, v]
+        R.ref_cscsym_to_coo.argtypes = [u32, u32, v, v, v, i, v, v, v]
+        _R = R
     return _R
 
 
