@@ -1,0 +1,404 @@
+// Batched JPCG kernels: S independent systems (one block-diagonal CSR), R
+// right-hand sides per system stored row-major [N][R].
+//
+// Work unit: a tile of consecutive rows of one system.  Inside a workgroup,
+// R lanes form a group that owns one row at a time -- lane j carries
+// right-hand side j -- so a row's column index and value are read once and
+// broadcast to the group while the R gathered x-values of a neighbour are one
+// contiguous 8R-byte segment.  Per-(system, rhs) reductions go tile partial ->
+// system total in fixed order (one workgroup per system), so every pair's
+// alpha / beta are reproducible.
+//
+//   bsetup / bsys_setup      r = b, J = 1/diag, p = J b, x = 0; ||b||^2, rz per pair
+//   bspmm_dot                Ap = A p, p.Ap tile partials          (one workgroup per tile)
+//   bsys_alpha               alpha per pair                        (one workgroup per system)
+//   bupdate_r                r -= alpha Ap; r.r, r.z tile partials
+//   bsys_beta                beta, stop rule and masks per pair
+//   bupdate_xp               x += alpha p; p = beta p + J r        (pairs that ran this iteration)
+//   bfinish                  iterations and relative residual per pair
+#include "batched.hpp"
+#include "kernels.hpp"
+
+namespace xj {
+
+constexpr int kMaxBlock = 512;
+
+// Sum over the row groups of a workgroup for each lane j < R; lanes with
+// g == 0 return the total (fixed order g = 0, 1, ...).
+template <int R>
+__device__ __forceinline__ double group_sum(double v, double* red) {
+    const int j = threadIdx.x % R, g = threadIdx.x / R, G = blockDim.x / R;
+    red[g * R + j] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (g == 0)
+        for (int q = 0; q < G; ++q) s += red[q * R + j];
+    __syncthreads();
+    return s;
+}
+
+// Sum of part[t*R + j] over the tiles t of system s, fixed order.
+template <int R>
+__device__ __forceinline__ double system_sum(const double* part, int t0, int t1, double* red) {
+    const int j = threadIdx.x % R, g = threadIdx.x / R, G = blockDim.x / R;
+    double s = 0.0;
+    for (int t = t0 + g; t < t1; t += G) s += part[(size_t)t * R + j];
+    return group_sum<R>(s, red);
+}
+
+template <int R>
+__global__ __launch_bounds__(kMaxBlock) void bsetup(BatchParams P) {
+    __shared__ double red[kMaxBlock];
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int j = threadIdx.x % R, g = threadIdx.x / R, G = blockDim.x / R;
+    const int r0 = P.tile_row[t], r1 = P.tile_row[t + 1];
+    double dot = 0.0, rz = 0.0;
+    for (int i = r0 + g; i < r1; i += G) {
+        const size_t e = (size_t)i * R + j;
+        const double bi = P.b[e];
+        const double Ji = 1.0 / P.diag[i];
+        const double zi = Ji * bi;
+        if (j == 0) P.J[i] = Ji;
+        P.r[e] = bi;
+        P.p[e] = zi;
+        P.x[e] = 0.0;
+        dot += bi * bi;
+        rz += bi * zi;
+    }
+    dot = group_sum<R>(dot, red);
+    rz = group_sum<R>(rz, red);
+    if (g == 0) {
+        P.partB[(size_t)t * R + j] = dot;
+        P.partC[(size_t)t * R + j] = rz;
+    }
+}
+
+template <int R>
+__global__ __launch_bounds__(kMaxBlock) void bsys_setup(BatchParams P) {
+    __shared__ double red[kMaxBlock];
+    __shared__ int any;
+    const int s = blockIdx.x;
+    const int j = threadIdx.x % R, g = threadIdx.x / R;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    const int t0 = P.sys_tile[s], t1 = P.sys_tile[s + 1];
+    const double dot = system_sum<R>(P.partB, t0, t1, red);
+    const double rz = system_sum<R>(P.partC, t0, t1, red);
+    if (g == 0) {
+        const int q = s * R + j;
+        P.dot[q] = dot;
+        P.tols[q] = dot * P.tol * P.tol;
+        P.rz[q] = rz;
+        P.rr[q] = dot;
+        P.alpha[q] = 0.0;
+        P.beta[q] = 0.0;
+        P.iters[q] = 0;
+        P.it[q] = 0;
+        const int a = dot > 0.0 ? 1 : 0;  // b == 0: x = 0, the pair never runs
+        P.active[q] = a;
+        P.upd[q] = 0;
+        if (a) atomicOr(&any, 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        P.sys_active[s] = any;
+        P.sys_upd[s] = 0;
+        if (any) *P.any_active = 1;
+    }
+}
+
+template <typename VT, int R>
+__global__ __launch_bounds__(kMaxBlock) void bspmm_dot(BatchParams P) {
+    __shared__ double red[kMaxBlock];
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int s = P.tile_sys[t];
+    if (!P.sys_active[s]) return;
+    const int j = threadIdx.x % R, g = threadIdx.x / R, G = blockDim.x / R;
+    const int r0 = P.tile_row[t], r1 = P.tile_row[t + 1];
+    const VT* __restrict__ val = static_cast<const VT*>(P.val);
+    const double* __restrict__ p = P.p;
+    double dl = 0.0;
+    for (int i = r0 + g; i < r1; i += G) {
+        const int k0 = P.rowptr[i], k1 = P.rowptr[i + 1];
+        double acc = P.dsplit ? P.dsplit[i] * p[(size_t)i * R + j] : 0.0;
+        for (int k = k0; k < k1; ++k) acc += (double)ld_stream(val + k) * p[(size_t)ld_stream(P.col + k) * R + j];
+        P.Ap[(size_t)i * R + j] = acc;
+        dl += p[(size_t)i * R + j] * acc;
+    }
+    dl = group_sum<R>(dl, red);
+    if (g == 0) P.partA[(size_t)t * R + j] = dl;
+}
+
+// plain batched SpMM y = A x (or y = b - A x) for the autotuner and the refinement residual
+template <typename VT, int R>
+__global__ __launch_bounds__(kMaxBlock) void bspmm(BatchParams P, const double* __restrict__ xin,
+                                                   double* __restrict__ y, const double* __restrict__ b) {
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int j = threadIdx.x % R, g = threadIdx.x / R, G = blockDim.x / R;
+    const int r0 = P.tile_row[t], r1 = P.tile_row[t + 1];
+    const VT* __restrict__ val = static_cast<const VT*>(P.val);
+    for (int i = r0 + g; i < r1; i += G) {
+        const int k0 = P.rowptr[i], k1 = P.rowptr[i + 1];
+        double acc = P.dsplit ? P.dsplit[i] * xin[(size_t)i * R + j] : 0.0;
+        for (int k = k0; k < k1; ++k)
+            acc += (double)ld_stream(val + k) * xin[(size_t)ld_stream(P.col + k) * R + j];
+        const size_t e = (size_t)i * R + j;
+        y[e] = b ? b[e] - acc : acc;
+    }
+}
+
+template <int R>
+__global__ __launch_bounds__(kMaxBlock) void bsys_alpha(BatchParams P) {
+    __shared__ double red[kMaxBlock];
+    const int s = blockIdx.x;
+    if (s == 0 && threadIdx.x == 0) *P.any_active = 0;  // bsys_beta sets it again for systems still running
+    if (!P.sys_active[s]) return;
+    const int j = threadIdx.x % R, g = threadIdx.x / R;
+    const double pap = system_sum<R>(P.partA, P.sys_tile[s], P.sys_tile[s + 1], red);
+    if (g == 0) {
+        const int q = s * R + j;
+        P.alpha[q] = P.active[q] ? P.rz[q] / pap : 0.0;
+    }
+}
+
+template <int R>
+__global__ __launch_bounds__(kMaxBlock) void bupdate_r(BatchParams P) {
+    __shared__ double red[kMaxBlock];
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int s = P.tile_sys[t];
+    if (!P.sys_active[s]) return;
+    const int j = threadIdx.x % R, g = threadIdx.x / R, G = blockDim.x / R;
+    const int r0 = P.tile_row[t], r1 = P.tile_row[t + 1];
+    const double alpha = P.alpha[s * R + j];
+    double rr = 0.0, rz = 0.0;
+    for (int i = r0 + g; i < r1; i += G) {
+        const size_t e = (size_t)i * R + j;
+        const double ri = -alpha * P.Ap[e] + P.r[e];
+        P.r[e] = ri;
+        const double zi = P.J[i] * ri;
+        rr += ri * ri;
+        rz += ri * zi;
+    }
+    rr = group_sum<R>(rr, red);
+    rz = group_sum<R>(rz, red);
+    if (g == 0) {
+        P.partB[(size_t)t * R + j] = rr;
+        P.partC[(size_t)t * R + j] = rz;
+    }
+}
+
+template <int R>
+__global__ __launch_bounds__(kMaxBlock) void bsys_beta(BatchParams P) {
+    __shared__ double red[kMaxBlock];
+    __shared__ int any_active, any_upd;
+    const int s = blockIdx.x;
+    const int j = threadIdx.x % R, g = threadIdx.x / R;
+    if (!P.sys_active[s]) {
+        if (g == 0) P.upd[s * R + j] = 0;
+        if (threadIdx.x == 0) P.sys_upd[s] = 0;
+        return;
+    }
+    if (threadIdx.x == 0) {
+        any_active = 0;
+        any_upd = 0;
+    }
+    __syncthreads();
+    const int t0 = P.sys_tile[s], t1 = P.sys_tile[s + 1];
+    const double rr = system_sum<R>(P.partB, t0, t1, red);
+    const double rzn = system_sum<R>(P.partC, t0, t1, red);
+    if (g == 0) {
+        const int q = s * R + j;
+        const int a = P.active[q];
+        P.upd[q] = a;
+        double beta = 0.0;
+        if (a) {
+            beta = rzn / P.rz[q];
+            P.rz[q] = rzn;
+            P.rr[q] = rr;
+            P.iters[q] = P.iters[q] + 1;
+            // stop rule of the reference control loop (control.hpp:83-88, tasks.hpp:43-47)
+            int it = P.it[q];
+            bool more = false;
+            if (!(rr < P.tols[q])) {
+                ++it;
+                more = (uint32_t)it < P.max_rounds;
+            }
+            P.it[q] = it;
+            if (!more) P.active[q] = 0;
+            else atomicOr(&any_active, 1);
+            atomicOr(&any_upd, 1);
+        }
+        P.beta[q] = beta;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        P.sys_active[s] = any_active;
+        P.sys_upd[s] = any_upd;
+        if (any_active) *P.any_active = 1;
+    }
+}
+
+template <int R>
+__global__ __launch_bounds__(kMaxBlock) void bupdate_xp(BatchParams P) {
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int s = P.tile_sys[t];
+    if (!P.sys_upd[s]) return;
+    const int j = threadIdx.x % R, g = threadIdx.x / R, G = blockDim.x / R;
+    const int q = s * R + j;
+    if (!P.upd[q]) return;
+    const int r0 = P.tile_row[t], r1 = P.tile_row[t + 1];
+    const double alpha = P.alpha[q], beta = P.beta[q];
+    for (int i = r0 + g; i < r1; i += G) {
+        const size_t e = (size_t)i * R + j;
+        const double pi = P.p[e];
+        P.x[e] = alpha * pi + P.x[e];
+        P.p[e] = beta * pi + P.J[i] * P.r[e];
+    }
+}
+
+__global__ void bfinish(int nq, const double* __restrict__ dot, const double* __restrict__ rr,
+                        const int* __restrict__ iters, int* __restrict__ out_iters, double* __restrict__ out_res) {
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+        if (out_iters) out_iters[q] = iters[q];
+        if (out_res) out_res[q] = dot[q] > 0.0 ? sqrt(rr[q] / dot[q]) : 0.0;
+    }
+}
+
+// per-pair ||v||^2 over each system (for the refinement's true residual)
+template <int R>
+__global__ __launch_bounds__(kMaxBlock) void btile_norm(BatchParams P, const double* __restrict__ v,
+                                                        double* __restrict__ part) {
+    __shared__ double red[kMaxBlock];
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int j = threadIdx.x % R, g = threadIdx.x / R, G = blockDim.x / R;
+    double s = 0.0;
+    for (int i = P.tile_row[t] + g; i < P.tile_row[t + 1]; i += G) {
+        const double a = v[(size_t)i * R + j];
+        s += a * a;
+    }
+    s = group_sum<R>(s, red);
+    if (g == 0) part[(size_t)t * R + j] = s;
+}
+
+template <int R>
+__global__ __launch_bounds__(kMaxBlock) void bsys_norm(BatchParams P, const double* __restrict__ part,
+                                                       double* __restrict__ out) {
+    __shared__ double red[kMaxBlock];
+    const int s = blockIdx.x;
+    const int j = threadIdx.x % R, g = threadIdx.x / R;
+    const double v = system_sum<R>(part, P.sys_tile[s], P.sys_tile[s + 1], red);
+    if (g == 0) out[s * R + j] = v;
+}
+
+// v[e] = 0 for the pairs whose flag is 0 (masks converged pairs out of a refinement step)
+template <int R>
+__global__ __launch_bounds__(kMaxBlock) void bmask(BatchParams P, const int* __restrict__ keep,
+                                                   double* __restrict__ v) {
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int s = P.tile_sys[t];
+    const int j = threadIdx.x % R, g = threadIdx.x / R, G = blockDim.x / R;
+    if (keep[s * R + j]) return;
+    for (int i = P.tile_row[t] + g; i < P.tile_row[t + 1]; i += G) v[(size_t)i * R + j] = 0.0;
+}
+
+__global__ void baxpy(size_t m, const double* __restrict__ d, double* __restrict__ x) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += (size_t)gridDim.x * blockDim.x)
+        x[e] += d[e];
+}
+
+// --------------------------------------------------------------------------
+// launchers
+
+template <int R>
+static void iteration_r(const BatchParams& P, bool f32, int block, hipStream_t s) {
+    if (f32)
+        bspmm_dot<float, R><<<P.T, block, 0, s>>>(P);
+    else
+        bspmm_dot<double, R><<<P.T, block, 0, s>>>(P);
+    bsys_alpha<R><<<P.S, 256, 0, s>>>(P);
+    bupdate_r<R><<<P.T, block, 0, s>>>(P);
+    bsys_beta<R><<<P.S, 256, 0, s>>>(P);
+    bupdate_xp<R><<<P.T, block, 0, s>>>(P);
+}
+
+template <int R>
+static void setup_r(const BatchParams& P, int block, hipStream_t s) {
+    bsetup<R><<<P.T, block, 0, s>>>(P);
+    bsys_setup<R><<<P.S, 256, 0, s>>>(P);
+}
+
+template <int R>
+static void spmm_r(const BatchParams& P, bool f32, int block, const double* x, double* y, const double* b,
+                   hipStream_t s) {
+    if (f32)
+        bspmm<float, R><<<P.T, block, 0, s>>>(P, x, y, b);
+    else
+        bspmm<double, R><<<P.T, block, 0, s>>>(P, x, y, b);
+}
+
+template <int R>
+static void norms_r(const BatchParams& P, int block, const double* v, double* part, double* out, hipStream_t s) {
+    btile_norm<R><<<P.T, block, 0, s>>>(P, v, part);
+    bsys_norm<R><<<P.S, 256, 0, s>>>(P, part, out);
+}
+
+template <int R>
+static void mask_r(const BatchParams& P, int block, const int* keep, double* v, hipStream_t s) {
+    bmask<R><<<P.T, block, 0, s>>>(P, keep, v);
+}
+
+#define XJ_DISPATCH_R(R_, CALL)                       \
+    switch (R_) {                                     \
+        case 1: { constexpr int RR = 1; CALL; break; }  \
+        case 2: { constexpr int RR = 2; CALL; break; }  \
+        case 4: { constexpr int RR = 4; CALL; break; }  \
+        case 8: { constexpr int RR = 8; CALL; break; }  \
+        case 16: { constexpr int RR = 16; CALL; break; } \
+        case 32: { constexpr int RR = 32; CALL; break; } \
+        default: return hipErrorInvalidValue;         \
+    }
+
+hipError_t batch_launch_setup(const BatchParams& P, int R, int block, hipStream_t s) {
+    XJ_DISPATCH_R(R, setup_r<RR>(P, block, s));
+    return hipGetLastError();
+}
+
+hipError_t batch_launch_iteration(const BatchParams& P, int R, bool f32, int block, hipStream_t s) {
+    XJ_DISPATCH_R(R, iteration_r<RR>(P, f32, block, s));
+    return hipGetLastError();
+}
+
+hipError_t batch_launch_spmm(const BatchParams& P, int R, bool f32, int block, const double* x, double* y,
+                             const double* b, hipStream_t s) {
+    XJ_DISPATCH_R(R, spmm_r<RR>(P, f32, block, x, y, b, s));
+    return hipGetLastError();
+}
+
+hipError_t batch_launch_norms(const BatchParams& P, int R, int block, const double* v, double* part, double* out,
+                              hipStream_t s) {
+    XJ_DISPATCH_R(R, norms_r<RR>(P, block, v, part, out, s));
+    return hipGetLastError();
+}
+
+hipError_t batch_launch_mask(const BatchParams& P, int R, int block, const int* keep, double* v, hipStream_t s) {
+    XJ_DISPATCH_R(R, mask_r<RR>(P, block, keep, v, s));
+    return hipGetLastError();
+}
+
+hipError_t batch_launch_finish(int nq, const double* dot, const double* rr, const int* iters, int* out_iters,
+                               double* out_res, hipStream_t s) {
+    int g = (nq + 255) / 256;
+    if (g < 1) g = 1;
+    bfinish<<<g, 256, 0, s>>>(nq, dot, rr, iters, out_iters, out_res);
+    return hipGetLastError();
+}
+
+hipError_t batch_launch_axpy(size_t m, const double* d, double* x, hipStream_t s) {
+    size_t g = (m + 255) / 256;
+    if (g > 65536) g = 65536;
+    if (g < 1) g = 1;
+    baxpy<<<(unsigned)g, 256, 0, s>>>(m, d, x);
+    return hipGetLastError();
+}
+
+}  // namespace xj

@@ -1,0 +1,73 @@
+/*
+ * Batched JPCG (libxjpcg.so): many independent SPD systems of uneven size,
+ * each with R right-hand sides, solved together on one GPU.
+ *
+ * No reference counterpart: the reference's control unit can carry several
+ * tasks with per-task IDs and stop tests (pcg/hw/include/control.hpp:54-63,
+ * :83-88) but the product builds it with CG_numTasks=1 (pcg/hw/Makefile:103).
+ *
+ * Layout in HBM (all device pointers):
+ *   the S systems form one block-diagonal CSR: d_rowptr[N+1], d_col[nnz]
+ *   (global column indices), d_val[nnz] (fp64 or fp32), optional split fp64
+ *   diagonal d_dsplit[N] (the CSR then holds off-diagonal entries only);
+ *   h_sys_row[S+1] (host) gives each system's row range;
+ *   vectors are row-major [N][R]: the R right-hand sides of a row are contiguous.
+ * Each (system, rhs) pair runs the reference's JPCG semantics (x0 = 0, Jacobi
+ * with the caller's diagonal d_diag[N], stop when ||r||^2 < tol^2 ||b||^2 or
+ * after maxIter+1 iterations) with its own alpha, beta and convergence mask;
+ * finished pairs stop updating and finished systems drop out of the launches.
+ */
+#ifndef XJPCG_BATCH_H
+#define XJPCG_BATCH_H
+
+#include <stdint.h>
+
+#include "pcg.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct xjpcg_batch xjpcg_batch_t;
+
+/* message of the last failing batch call on this thread */
+const char* xjpcg_batch_last_error(void);
+
+/* nrhs must be 1, 2, 4, 8, 16 or 32.  h_rowptr (host copy of d_rowptr) is required. */
+XJPCG_Status_t xjpcg_batch_create(xjpcg_batch_t** batch, int32_t nsys, int32_t nrhs, const int32_t* h_sys_row,
+                                  const int32_t* h_rowptr, const int32_t* d_rowptr, const int32_t* d_col,
+                                  const void* d_val, int val_f32, const double* d_dsplit, void* stream);
+XJPCG_Status_t xjpcg_batch_destroy(xjpcg_batch_t* batch);
+
+/* rows N, stored entries, tiles, rows per tile, threads per workgroup */
+XJPCG_Status_t xjpcg_batch_info(const xjpcg_batch_t* batch, int32_t* nrows, int32_t* nnz, int32_t* ntiles,
+                                int32_t* tile_rows, int32_t* block);
+
+/* Launch configuration: tile_rows (rows per tile, 16..1024) and block (threads
+ * per workgroup: 64, 128, 256 or 512); tune != 0 times the candidate
+ * configurations on this batch's SpMM and keeps the fastest (the autotuner),
+ * ignoring the other two arguments. */
+XJPCG_Status_t xjpcg_batch_set_config(xjpcg_batch_t* batch, int32_t tile_rows, int32_t block, int32_t tune);
+
+/* Solve every (system, rhs) pair from x0 = 0.  d_b, d_x: [N][R].
+ * d_iters (int32 [S][R], may be NULL) receives the iterations each pair ran,
+ * d_res (fp64 [S][R], may be NULL) its final ||r||/||b|| (recursive residual).
+ * *h_launched (may be NULL) = batched iterations launched. */
+XJPCG_Status_t xjpcg_batch_solve(xjpcg_batch_t* batch, const double* d_diag, const double* d_b, double* d_x,
+                                 uint32_t max_iter, double tol, int32_t* d_iters, double* d_res,
+                                 int64_t* h_launched, void* stream);
+
+/* Iterative refinement: `outer` (fp64-accurate values, same pattern and
+ * sizes) computes the true residual r = b - A x in fp64; `inner` (typically
+ * fp32 values) solves A d = r to inner_tol; x += d; repeat until every pair
+ * has ||r||/||b|| < tol or max_outer steps.  d_res (fp64 [S][R], may be NULL)
+ * receives the final true relative residuals. */
+XJPCG_Status_t xjpcg_batch_solve_mixed(xjpcg_batch_t* outer, xjpcg_batch_t* inner, const double* d_diag,
+                                       const double* d_b, double* d_x, uint32_t max_outer, double tol,
+                                       double inner_tol, uint32_t inner_max_iter, int32_t* d_iters, double* d_res,
+                                       uint32_t* h_outer, int64_t* h_inner_launched, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,117 @@
+"""Batched JPCG (many uneven systems x R right-hand sides) against the CPU oracle, per pair."""
+import sys
+
+import numpy as np
+import pytest
+
+from hpc_amd import workloads
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(m):
+    print(m, flush=True)
+    sys.stdout.flush()
+
+
+def small_batch(count=6, nrhs=4, nmin=300, nmax=3000, seed=3):
+    mats, rhs = workloads.c4_systems(count, nmin, nmax, nrhs, seed)
+    return mats, rhs
+
+
+def _dev(cuda, a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
+
+
+def test_batched_fp64_matches_oracle_per_pair(cuda):
+    from hpc_amd import batched
+    mats, rhs = small_batch()
+    R = rhs[0].shape[1]
+    asm = batched.assemble(mats)
+    B = batched.Batch.from_host(asm, R, cuda)
+    _say(f"batch S={B.S} N={B.N} R={R} info={B.info()}")
+    b = _dev(cuda, np.concatenate(rhs))
+    diag = _dev(cuda, asm["diag"])
+    tol, maxit = 1e-8, 5000
+    x, info = B.solve(diag, b, maxit, tol)
+    xh = x.cpu().numpy()
+    its = info.iterations.cpu().numpy()
+    res = info.residual.cpu().numpy()
+    for s, A in enumerate(mats):
+        r0, r1 = asm["sys_row"][s], asm["sys_row"][s + 1]
+        for j in range(R):
+            ref = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, rhs[s][:, j].copy(), maxit, tol)
+            assert abs(int(its[s, j]) - ref["executed"]) <= 1, (s, j, its[s, j], ref["executed"])
+            e = np.linalg.norm(xh[r0:r1, j] - ref["x"]) / np.linalg.norm(ref["x"])
+            assert e <= 1e-10, (s, j, e)
+            assert res[s, j] < tol
+    assert info.launched >= its.max()
+
+
+def test_batched_zero_rhs_and_masks(cuda):
+    from hpc_amd import batched
+    mats, rhs = small_batch(count=3, nrhs=2)
+    rhs[1][:, 0] = 0.0
+    asm = batched.assemble(mats)
+    B = batched.Batch.from_host(asm, 2, cuda)
+    x, info = B.solve(_dev(cuda, asm["diag"]), _dev(cuda, np.concatenate(rhs)), 4000, 1e-8)
+    its = info.iterations.cpu().numpy()
+    assert its[1, 0] == 0
+    xh = x.cpu().numpy()
+    assert not xh[asm["sys_row"][1]:asm["sys_row"][2], 0].any()
+    assert its[1, 1] > 0
+
+
+def test_batched_iteration_cap(cuda):
+    from hpc_amd import batched
+    mats, rhs = small_batch(count=2, nrhs=1)
+    asm = batched.assemble(mats)
+    B = batched.Batch.from_host(asm, 1, cuda)
+    x, info = B.solve(_dev(cuda, asm["diag"]), _dev(cuda, np.concatenate(rhs)), 7, 1e-12)
+    assert np.all(info.iterations.cpu().numpy() == 8)  # maxIter+1 rounds, as the reference control loop
+
+
+def test_batched_autotune_keeps_results(cuda):
+    from hpc_amd import batched
+    mats, rhs = small_batch(count=4, nrhs=8)
+    asm = batched.assemble(mats)
+    B = batched.Batch.from_host(asm, 8, cuda)
+    b, d = _dev(cuda, np.concatenate(rhs)), _dev(cuda, asm["diag"])
+    x1, i1 = B.solve(d, b, 5000, 1e-8)
+    x1 = x1.clone()
+    B.configure(tune=True)
+    _say(f"autotuned: {B.info()}")
+    x2, i2 = B.solve(d, b, 5000, 1e-8)
+    assert np.array_equal(i1.iterations.cpu().numpy(), i2.iterations.cpu().numpy()) or \
+        np.abs(i1.iterations.cpu().numpy() - i2.iterations.cpu().numpy()).max() <= 1
+    assert float(((x1 - x2).norm() / x1.norm()).item()) < 1e-10
+    x3, i3 = B.solve(d, b, 5000, 1e-8)
+    assert bool((x2 == x3).all())  # deterministic for a fixed configuration
+
+
+def test_batched_mixed_refined(cuda):
+    from hpc_amd import batched
+    mats, rhs = small_batch(count=5, nrhs=4, seed=8)
+    asm64 = batched.assemble(mats)
+    asm32 = batched.assemble(mats, split_fp32=True)
+    assert asm32["exact"]
+    outer = batched.Batch.from_host(asm32, 4, cuda)            # fp32 values are exact: the split operator is A
+    inner = batched.Batch.from_host(asm32, 4, cuda)
+    inner.configure(tune=True)
+    b, d = _dev(cuda, np.concatenate(rhs)), _dev(cuda, asm64["diag"])
+    tol = 1e-8
+    x, info, steps = batched.solve_mixed(outer, inner, d, b, tol, inner_tol=1e-4)
+    _say(f"mixed batched: outer steps {steps}, launched {info.launched}")
+    assert steps >= 2  # inner 1e-4 cannot reach 1e-8 in one step
+    res = info.residual.cpu().numpy()
+    assert res.max() < tol
+    xh = x.cpu().numpy()
+    for s, A in enumerate(mats):
+        r0, r1 = asm64["sys_row"][s], asm64["sys_row"][s + 1]
+        M = A.to_scipy()
+        for j in range(4):
+            bj = rhs[s][:, j]
+            true = np.linalg.norm(bj - M @ xh[r0:r1, j]) / np.linalg.norm(bj)
+            assert true < tol * 1.01, (s, j, true)
