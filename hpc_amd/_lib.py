@@ -36,6 +36,7 @@ _SIGNATURES = {
     "xjpcg_op_create": (c_int, [POINTER(c_void_p), c_int32, c_int64, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
                                 c_void_p, c_void_p]),
     "xjpcg_op_destroy": (c_int, [c_void_p]),
+    "xjpcg_op_set_values": (c_int, [c_void_p, c_void_p]),
     "xjpcg_op_info": (c_int, [c_void_p, POINTER(c_int32), POINTER(c_int64), POINTER(c_int32), POINTER(c_int32)]),
     "xjpcg_op_set_path": (c_int, [c_void_p, c_int, POINTER(c_int)]),
     "xjpcg_spmv": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),

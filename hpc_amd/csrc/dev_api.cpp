@@ -61,6 +61,10 @@ XJPCG_Status_t xjpcg_op_destroy(xjpcg_op_t* op) {
     return XJPCG_STATUS_SUCCESS;
 }
 
+XJPCG_Status_t xjpcg_op_set_values(xjpcg_op_t* op, const void* d_val) {
+    return guarded([&] { impl_of(op).set_values(d_val); });
+}
+
 XJPCG_Status_t xjpcg_op_info(const xjpcg_op_t* op, int32_t* n, int64_t* nnz, int32_t* ntiles, int32_t* grid) {
     return guarded([&] {
         const xj::Operator& o = impl_of(const_cast<xjpcg_op_t*>(op));

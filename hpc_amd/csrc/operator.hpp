@@ -76,6 +76,12 @@ class Operator {
 
     const CsrDev& csr() const { return A_; }
     int grid() const { return G_; }
+    // point the operator at another value array of the same type and length
+    // (the pattern and tile plan stay); used when only A's values change
+    void set_values(const void* val) {
+        if (val == nullptr && A_.nnz > 0) throw invalid("Matrix is nullptr.");
+        A_.val = val;
+    }
 
     void spmv(const double* x, double* y, hipStream_t s) const;
     // r = b - A x; returns r.r (synchronises the stream)

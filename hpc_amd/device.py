@@ -90,6 +90,14 @@ class CsrOperator:
                                       ctypes.byref(g)))
         return {"n": n.value, "nnz": nnz.value, "ntiles": nt.value, "grid": g.value}
 
+    def set_values(self, val: torch.Tensor):
+        """Point the operator at new values (same dtype and length); pattern and tiles are kept."""
+        if val.dtype != self.val.dtype:
+            raise ValueError("values must keep their dtype")
+        _dev_check(val, val.dtype, "val", self.nnz)
+        check_dev(lib().xjpcg_op_set_values(self._h, ptr(val)))
+        self.val = val
+
     PATH_AUTO, PATH_PERSISTENT, PATH_MULTI = 0, 1, 2
 
     def set_path(self, path: int):

@@ -160,6 +160,64 @@ def c3_graph(n: int = 4_000_000, seed: int = 2) -> System:
     return System(A, b, 1e-10, 100000, None, f"graph_{n}")
 
 
+def gaussian_random_field(m: int, length: float, seed: int) -> np.ndarray:
+    """Stationary Gaussian random field on an m x m periodic grid, squared-exponential covariance
+    exp(-d^2 / (2 length^2)), variance 1: white noise (numpy PCG64 seeded with `seed`) filtered in
+    Fourier space by the square root of the covariance spectrum, then scaled to unit sample variance."""
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal((m, m))
+    k = np.fft.fftfreq(m) * 2 * np.pi
+    kk = k[:, None] ** 2 + k[None, :] ** 2
+    spec = np.exp(-0.5 * kk * length ** 2)  # spectrum of the squared-exponential kernel (up to a constant)
+    g = np.real(np.fft.ifft2(np.fft.fft2(w) * np.sqrt(spec)))
+    g -= g.mean()
+    return g / g.std()
+
+
+def diffusion2d(k_face_x: np.ndarray, k_face_y: np.ndarray) -> CsrMatrix:
+    """Cell-centred 2D diffusion on an m x m grid with Dirichlet walls.
+
+    k_face_x[i, j] is the conductivity of the face on the -x side of cell (row i, column j) for
+    j = 0..m (shape m x (m+1)); k_face_y likewise on the -y side (shape (m+1) x m).  Interior face
+    between cells a, b with conductivity k adds k to A_aa and A_bb and -k to A_ab, A_ba; a wall
+    face adds its k to the adjacent cell's diagonal.  Rows are i*m + j, columns ascending."""
+    m = k_face_x.shape[0]
+    n = m * m
+    diag = (k_face_x[:, :-1] + k_face_x[:, 1:] + k_face_y[:-1, :] + k_face_y[1:, :]).ravel()
+    rows, cols, vals = [np.arange(n)], [np.arange(n)], [diag]
+    idx = np.arange(n).reshape(m, m)
+    # x faces between (i, j-1) and (i, j), j = 1..m-1
+    a, b = idx[:, :-1].ravel(), idx[:, 1:].ravel()
+    kx = k_face_x[:, 1:-1].ravel()
+    ay, by = idx[:-1, :].ravel(), idx[1:, :].ravel()
+    ky = k_face_y[1:-1, :].ravel()
+    rows += [a, b, ay, by]
+    cols += [b, a, by, ay]
+    vals += [-kx, -kx, -ky, -ky]
+    import scipy.sparse as sp
+    M = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+    M.sort_indices()
+    return CsrMatrix(M.indptr.astype(np.int32), M.indices.astype(np.int32), M.data.astype(np.float64), diag.copy(),
+                     {"kind": "diffusion2d", "m": m})
+
+
+def c5_diffusion(m: int = 512, length: float = 16.0, seed: int = 4):
+    """2D variable-coefficient diffusion m^2 (default 512^2): face conductivities exp(GRF) with
+    GRF length scale 16 cells, variance 1 (seed 4); b ~ U[-1,1] (seed 4, stream 1); x_target ~ U[-1,1]
+    (seed 4, stream 2).  Returns (CsrMatrix, b, x_target)."""
+    g = gaussian_random_field(m, length, seed)
+    gx = np.empty((m, m + 1))
+    gx[:, 1:-1] = 0.5 * (g[:, :-1] + g[:, 1:])
+    gx[:, 0], gx[:, -1] = g[:, 0], g[:, -1]
+    gy = np.empty((m + 1, m))
+    gy[1:-1, :] = 0.5 * (g[:-1, :] + g[1:, :])
+    gy[0, :], gy[-1, :] = g[0, :], g[-1, :]
+    A = diffusion2d(np.exp(gx), np.exp(gy))
+    b = uniform(seed, 1, A.n, -1.0, 1.0)
+    xt = uniform(seed, 2, A.n, -1.0, 1.0)
+    return A, b, xt
+
+
 def c4_systems(count: int = 512, nmin: int = 5000, nmax: int = 400_000, nrhs: int = 16, seed: int = 3):
     """Batched workload: `count` anisotropic diffusion systems (2D 5-pt or 3D 7-pt, coefficients
     exp(0.5 N(0,1)) rounded to fp32), n log-uniform in [nmin, nmax]; nrhs right-hand sides

@@ -41,6 +41,10 @@ XJPCG_Status_t xjpcg_op_create(xjpcg_op_t** op, int32_t n, int64_t nnz, const in
                                const int32_t* d_col, const void* d_val, int val_f32, const double* d_dsplit,
                                const int32_t* h_rowptr, void* stream);
 XJPCG_Status_t xjpcg_op_destroy(xjpcg_op_t* op);
+/* Rebind the operator to another device value array of the same type and
+ * length; the pattern and row tiles are kept (the device-side counterpart of
+ * the reference's KEEP_NZ_LAYOUT value refresh, signature.hpp:427-469). */
+XJPCG_Status_t xjpcg_op_set_values(xjpcg_op_t* op, const void* d_val);
 /* n, nnz, number of row tiles, grid of the persistent kernel */
 XJPCG_Status_t xjpcg_op_info(const xjpcg_op_t* op, int32_t* n, int64_t* nnz, int32_t* ntiles, int32_t* grid);
 
