@@ -68,7 +68,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=160)
     ap.add_argument("--spmv-repeats", type=int, default=100)
-    ap.add_argument("--inner-tol", type=float, default=1e-6)
+    ap.add_argument("--inner-tol", type=float, default=0.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -37,6 +37,7 @@ _SIGNATURES = {
                                 c_void_p, c_void_p]),
     "xjpcg_op_destroy": (c_int, [c_void_p]),
     "xjpcg_op_set_values": (c_int, [c_void_p, c_void_p]),
+    "xjpcg_op_set_compact": (c_int, [c_void_p, c_int, POINTER(c_int)]),
     "xjpcg_op_info": (c_int, [c_void_p, POINTER(c_int32), POINTER(c_int64), POINTER(c_int32), POINTER(c_int32)]),
     "xjpcg_op_set_path": (c_int, [c_void_p, c_int, POINTER(c_int)]),
     "xjpcg_spmv": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),

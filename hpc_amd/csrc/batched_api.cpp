@@ -438,9 +438,15 @@ extern "C" XJPCG_Status_t xjpcg_batch_solve_mixed(xjpcg_batch_t* outer, xjpcg_ba
                 left += keep[q];
             }
             if (left == 0 || steps >= max_outer) break;
+            // relative target for the correction: the unfinished pair furthest from tol sets it
+            // (see xjpcg_solve_mixed), bounded below by inner_tol and above by 0.5
+            double tin = 0.5;
+            for (int q = 0; q < SR; ++q)
+                if (keep[q]) tin = std::min(tin, 0.5 * tol / res[q]);
+            tin = std::min(std::max(tin, inner_tol), 0.5);
             xj::hip_check(hipMemcpyAsync(d_keep, keep.data(), SR * sizeof(int), hipMemcpyHostToDevice, s), "mask");
             O.mask(d_keep, r, s);  // finished pairs get a zero right-hand side: they do not run
-            launched += I.solve(d_diag, r, d, inner_max_iter, inner_tol, d_it, nullptr, s);
+            launched += I.solve(d_diag, r, d, inner_max_iter, tin, d_it, nullptr, s);
             xj::hip_check(xj::batch_launch_axpy(NR, d, d_x, s), "x += d");
             O.spmm(d_x, r, d_b, s);
             O.norms(r, d_rr, s);

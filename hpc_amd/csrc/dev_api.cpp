@@ -1,6 +1,7 @@
 // extern "C" device-level ABI (include/xjpcg_dev.h) over the C++ engine core.
 #include "xjpcg_dev.h"
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 
@@ -59,6 +60,14 @@ XJPCG_Status_t xjpcg_op_destroy(xjpcg_op_t* op) {
     delete op->impl;
     delete op;
     return XJPCG_STATUS_SUCCESS;
+}
+
+XJPCG_Status_t xjpcg_op_set_compact(xjpcg_op_t* op, int on, int* h_active) {
+    return guarded([&] {
+        xj::Operator& o = impl_of(op);
+        if (on >= 0) o.set_compact(on != 0);
+        if (h_active) *h_active = o.compact() ? 1 : 0;
+    });
 }
 
 XJPCG_Status_t xjpcg_op_set_values(xjpcg_op_t* op, const void* d_val) {
@@ -148,7 +157,13 @@ XJPCG_Status_t xjpcg_solve_mixed(xjpcg_op_t* outer, xjpcg_op_t* inner, const dou
         while (true) {
             res = std::sqrt(rr / bb);
             if (res < tol || outer_done >= max_outer) break;
-            xj::SolveResult R = I.solve(d_diag, r, d, inner_max_iter, inner_tol, nullptr, s);
+            // The correction only has to bring ||r|| below tol ||b||: relative
+            // target 0.5 tol ||b|| / ||r||, but never tighter than requested by
+            // inner_tol (which bounds the work per step when the inner operator
+            // is only an approximation of A) and never looser than 0.5.
+            double tin = std::max(inner_tol, 0.5 * tol / res);
+            tin = std::min(tin, 0.5);
+            xj::SolveResult R = I.solve(d_diag, r, d, inner_max_iter, tin, nullptr, s);
             inner_total += R.executed;
             xj::hip_check(xj::launch_axpy(n, 1.0, d, d_x, s), "x += d");
             rr = O.residual(d_b, d_x, r, s);

@@ -4,10 +4,15 @@
 //                    CG iteration and the per-iteration records, separated by
 //                    grid barriers (the reference likewise runs the whole loop
 //                    on the card from one host launch, pcg/sw/src/cgHost.cpp:447-471)
+//  cg_*              the same loop as three or four launches per iteration
 //  spmv_tiles        standalone y = A x (or y = b - A x), one workgroup per tile
+//  build_col16       compact int16 column stream (col - first row of the tile)
 //  dot_partials / sum_partials   fixed-order two-pass dot product
 //  gather_values     KEEP_NZ_LAYOUT value refresh through the stored permutation
 //  sddmm_tiles       g_k = -lam[row_k] * x[col_k] on the sparsity pattern
+//
+// Kernels that read the matrix are instantiated for fp64 / fp32 values (VT)
+// and int32 / int16-delta columns (CT).
 #include "kernels.hpp"
 #include "launch.hpp"
 
@@ -16,7 +21,7 @@ namespace xj {
 // --------------------------------------------------------------------------
 // persistent JPCG
 
-template <typename VT>
+template <typename VT, typename CT>
 __global__ __launch_bounds__(kBlock) void jpcg_persistent(JpcgParams P) {
     __shared__ double lds[kTileNnz];
     __shared__ double red[kWaves];
@@ -28,6 +33,7 @@ __global__ __launch_bounds__(kBlock) void jpcg_persistent(JpcgParams P) {
     const int gthreads = G * kBlock;
     const int n = P.n;
     const VT* __restrict__ val = static_cast<const VT*>(P.val);
+    const CT* __restrict__ col = static_cast<const CT*>(P.col);
     double* __restrict__ x = P.x;
     double* __restrict__ r = P.r;
     double* __restrict__ p = P.p;
@@ -81,8 +87,8 @@ __global__ __launch_bounds__(kBlock) void jpcg_persistent(JpcgParams P) {
             for (int t = sw.first; t < sw.end; t += sw.step) {
                 double y;
                 int row;
-                tile_spmv<VT>(P.tile_row[t], P.tile_row[t + 1], P.rowptr, P.col, val, P.dsplit, p, lds, red, &y,
-                              &row);
+                tile_spmv<VT, CT>(P.tile_row[t], P.tile_row[t + 1], P.rowptr, col, val, P.dsplit, p, lds, red, &y,
+                                  &row);
                 if (row >= 0) {
                     Ap[row] = y;
                     pap += p[row] * y;
@@ -92,12 +98,9 @@ __global__ __launch_bounds__(kBlock) void jpcg_persistent(JpcgParams P) {
             if (tid == 0) part[L] = pap;
             if (!grid_barrier(P.bar, target, G, P.abortf, &sflag)) break;
             const double alpha = rz / grid_total(part, G, red);
-            // B: x += alpha p; r -= alpha Ap; z = J r; r.r and r.z partials
-            // (reference update_xk.hpp:53-67, update_rk_jacobi.hpp:82-110)
+            // B: r -= alpha Ap; z = J r; r.r and r.z partials (reference update_rk_jacobi.hpp:82-110)
             double rr_l = 0.0, rz_l = 0.0;
             for (int i = gtid; i < n; i += gthreads) {
-                const double pi = p[i];
-                x[i] = alpha * pi + x[i];
                 const double ri = -alpha * Ap[i] + r[i];
                 r[i] = ri;
                 const double zi = J[i] * ri;
@@ -129,9 +132,14 @@ __global__ __launch_bounds__(kBlock) void jpcg_persistent(JpcgParams P) {
                 ++it;
                 more = it < P.max_rounds;
             }
+            // C: x += alpha p (reference update_xk.hpp:53-67) with the p of this iteration, then
+            // p = beta p + z (update_pk.hpp:52-72); skipped after the last iteration
+            for (int i = gtid; i < n; i += gthreads) {
+                const double pi = p[i];
+                x[i] = alpha * pi + x[i];
+                if (more) p[i] = beta * pi + J[i] * r[i];
+            }
             if (!more) break;
-            // C: p = beta p + z (reference update_pk.hpp:52-72); x was updated in B, before p changes
-            for (int i = gtid; i < n; i += gthreads) p[i] = beta * p[i] + J[i] * r[i];
             if (!grid_barrier(P.bar, target, G, P.abortf, &sflag)) break;
         }
     }
@@ -145,18 +153,19 @@ __global__ __launch_bounds__(kBlock) void jpcg_persistent(JpcgParams P) {
 }
 
 // --------------------------------------------------------------------------
-// multi-kernel JPCG: three launches per iteration, coherence by kernel boundaries.
+// multi-kernel JPCG: coherence by kernel boundaries.
 //
-//   cg_spmv      Ap = A p, p.Ap partials               (G1 workgroups, contiguous tile ranges)
-//   cg_update_r  alpha = rz/pAp; r -= alpha Ap; r.r and r.z partials   (G2, grid-stride)
+//   cg_spmv      Ap = A p, p.Ap partial per tile           (one workgroup per tile)
+//   cg_alpha     alpha = rz / sum of the partials           (one workgroup; only when tiles are many)
+//   cg_update_r  r -= alpha Ap; r.r and r.z partials       (G2, grid-stride)
 //   cg_update_xp beta = rz'/rz; x += alpha p; p = beta p + J r; record; stop rule
 //
-// Every workgroup reduces the previous kernel's partials itself in one fixed
-// order, so all workgroups hold bit-identical alpha / beta.  The loop state
-// lives in CgState; each field is written by workgroup 0 of one kernel only
-// and read by the others only in later kernels, so no kernel reads a field
-// that is being written during its own launch.  Once `stop` is set the
-// remaining enqueued launches return immediately.
+// Partials are reduced in one fixed order, either once by cg_alpha or by
+// every workgroup of the vector kernels, so all workgroups hold bit-identical
+// alpha / beta.  The loop state lives in CgState; each field is written by
+// workgroup 0 of one kernel only and read by the others only in later
+// kernels, so no kernel reads a field that is being written during its own
+// launch.  Once `stop` is set the remaining enqueued launches return at once.
 
 __global__ __launch_bounds__(kBlock) void cg_setup(CgParams P) {
     __shared__ double red[kWaves];
@@ -194,6 +203,7 @@ __global__ __launch_bounds__(kBlock) void cg_setup_finish(CgParams P) {
         st->rz0 = rz;
         st->rz[0] = rz;
         st->rz[1] = 0.0;
+        st->alpha = 0.0;
         st->k = 0xFFFFFFFFu;  // the first cg_spmv advances it to 0
         st->it = 0;
         st->stop = dot > 0.0 ? 0 : 1;  // b == 0: x = 0, no iteration
@@ -208,30 +218,47 @@ __global__ __launch_bounds__(kBlock) void cg_setup_finish(CgParams P) {
     }
 }
 
-template <typename VT>
+template <typename VT, typename CT>
 __global__ __launch_bounds__(kBlock) void cg_spmv(CgParams P) {
     __shared__ double lds[kTileNnz];
     __shared__ double red[kWaves];
     CgState* st = P.st;
     if (st->stop) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) st->k = st->k + 1;  // no other workgroup of this launch reads k
-    const int G = gridDim.x;
-    const int L = xcd_remap(blockIdx.x, G);
-    const TileSweep sw = tile_sweep(L, G, P.ntiles);
-    const VT* __restrict__ val = static_cast<const VT*>(P.val);
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
     const double* __restrict__ p = P.p;
+    double y;
+    int row;
+    tile_spmv<VT, CT>(P.tile_row[t], P.tile_row[t + 1], P.rowptr, static_cast<const CT*>(P.col),
+                      static_cast<const VT*>(P.val), P.dsplit, p, lds, red, &y, &row);
     double pap = 0.0;
-    for (int t = sw.first; t < sw.end; t += sw.step) {
-        double y;
-        int row;
-        tile_spmv<VT>(P.tile_row[t], P.tile_row[t + 1], P.rowptr, P.col, val, P.dsplit, p, lds, red, &y, &row);
-        if (row >= 0) {
-            P.Ap[row] = y;
-            pap += p[row] * y;
-        }
+    if (row >= 0) {
+        P.Ap[row] = y;
+        pap = p[row] * y;
     }
     pap = block_sum(pap, red);
-    if (threadIdx.x == 0) P.partA[L] = pap;
+    if (threadIdx.x == 0) P.partA[t] = pap;
+}
+
+__global__ __launch_bounds__(1024) void cg_alpha(CgParams P) {
+    __shared__ double red[16];
+    CgState* st = P.st;
+    if (st->stop) return;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < P.ntiles; i += 1024) s += P.partA[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = red[0];
+        for (int w = 1; w < 16; ++w) t += red[w];
+        st->alpha = st->rz[st->k & 1] / t;
+    }
+}
+
+__device__ __forceinline__ double cg_alpha_of(const CgParams& P, uint32_t k, double* red) {
+    if (P.split_alpha) return P.st->alpha;
+    return P.st->rz[k & 1] / grid_total(P.partA, P.ntiles, red);
 }
 
 __global__ __launch_bounds__(kBlock) void cg_update_r(CgParams P) {
@@ -241,8 +268,7 @@ __global__ __launch_bounds__(kBlock) void cg_update_r(CgParams P) {
     const int stop = st->stop;
     if (L == 0 && threadIdx.x == 0) st->stop_k2 = stop;
     if (stop) return;
-    const uint32_t k = st->k;
-    const double alpha = st->rz[k & 1] / grid_total(P.partA, P.G1, red);
+    const double alpha = cg_alpha_of(P, st->k, red);
     const int stride = gridDim.x * kBlock;
     double rr = 0.0, rz = 0.0;
     for (int i = L * kBlock + threadIdx.x; i < P.n; i += stride) {
@@ -267,7 +293,7 @@ __global__ __launch_bounds__(kBlock) void cg_update_xp(CgParams P) {
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t k = st->k;
     const double rz = st->rz[k & 1];
-    const double alpha = rz / grid_total(P.partA, P.G1, red);
+    const double alpha = cg_alpha_of(P, k, red);
     const double rr = grid_total(P.partB, P.G2, red);
     const double rzn = grid_total(P.partC, P.G2, red);
     const double beta = rzn / rz;
@@ -302,9 +328,9 @@ __global__ __launch_bounds__(kBlock) void cg_update_xp(CgParams P) {
 // --------------------------------------------------------------------------
 // standalone kernels
 
-template <typename VT>
+template <typename VT, typename CT>
 __global__ __launch_bounds__(kBlock) void spmv_tiles(int ntiles, const int* __restrict__ tile_row,
-                                                     const int* __restrict__ rowptr, const int* __restrict__ col,
+                                                     const int* __restrict__ rowptr, const CT* __restrict__ col,
                                                      const VT* __restrict__ val, const double* __restrict__ dsplit,
                                                      const double* __restrict__ x, double* __restrict__ y,
                                                      const double* __restrict__ b) {
@@ -314,8 +340,25 @@ __global__ __launch_bounds__(kBlock) void spmv_tiles(int ntiles, const int* __re
     if (t >= ntiles) return;
     double v;
     int row;
-    tile_spmv<VT>(tile_row[t], tile_row[t + 1], rowptr, col, val, dsplit, x, lds, red, &v, &row);
+    tile_spmv<VT, CT>(tile_row[t], tile_row[t + 1], rowptr, col, val, dsplit, x, lds, red, &v, &row);
     if (row >= 0) y[row] = b ? b[row] - v : v;
+}
+
+// col16[k] = col[k] - first row of k's tile; *overflow = 1 if any difference leaves int16
+__global__ __launch_bounds__(kBlock) void build_col16(int ntiles, const int* __restrict__ tile_row,
+                                                      const int* __restrict__ rowptr, const int* __restrict__ col,
+                                                      int16_t* __restrict__ col16, int* __restrict__ overflow) {
+    const int t = blockIdx.x;
+    if (t >= ntiles) return;
+    const int r0 = tile_row[t];
+    const int k0 = rowptr[r0], k1 = rowptr[tile_row[t + 1]];
+    int bad = 0;
+    for (int k = k0 + threadIdx.x; k < k1; k += kBlock) {
+        const int d = col[k] - r0;
+        if (d < -32768 || d > 32767) bad = 1;
+        col16[k] = (int16_t)d;
+    }
+    if (bad) atomicOr(overflow, 1);
 }
 
 // part[blk] = sum over this block's grid-stride slice of a[i]*b[i]
@@ -342,7 +385,7 @@ __global__ __launch_bounds__(kBlock) void gather_values(int64_t nnz, const int64
         dst[k] = (TO)src[perm[k]];
 }
 
-// y = x + a*d  (iterative-refinement update)
+// x = x + a*d  (iterative-refinement update)
 __global__ __launch_bounds__(kBlock) void axpy_kernel(int n, double a, const double* __restrict__ d,
                                                       double* __restrict__ x) {
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) x[i] = a * d[i] + x[i];
@@ -374,11 +417,25 @@ __global__ __launch_bounds__(kBlock) void sddmm_tiles(int ntiles, const int* __r
 // --------------------------------------------------------------------------
 // launchers
 
-static int g_coop_grid[2] = {0, 0};  // cached co-resident grid size per value type (single device)
+// Runs CALL with VT (value type) and CT (column type) bound to the operator's storage.
+#define XJ_DISPATCH_VC(F32, C16, CALL)                         \
+    do {                                                       \
+        if (F32) {                                             \
+            using VT = float;                                  \
+            if (C16) { using CT = int16_t; CALL; }             \
+            else { using CT = int; CALL; }                     \
+        } else {                                               \
+            using VT = double;                                 \
+            if (C16) { using CT = int16_t; CALL; }             \
+            else { using CT = int; CALL; }                     \
+        }                                                      \
+    } while (0)
 
-template <typename VT>
+static int g_coop_grid[4] = {0, 0, 0, 0};  // cached co-resident grid per (value, column) type (one device)
+
+template <typename VT, typename CT>
 static hipError_t coop_grid(int* G) {
-    const int slot = sizeof(VT) == 4 ? 1 : 0;
+    const int slot = (sizeof(VT) == 4 ? 1 : 0) + (sizeof(CT) == 2 ? 2 : 0);
     if (g_coop_grid[slot] > 0) {
         *G = g_coop_grid[slot];
         return hipSuccess;
@@ -388,7 +445,7 @@ static hipError_t coop_grid(int* G) {
     if (e != hipSuccess) return e;
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return e;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, jpcg_persistent<VT>, kBlock, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, jpcg_persistent<VT, CT>, kBlock, 0);
     if (e != hipSuccess) return e;
     if (per < 1) return hipErrorCooperativeLaunchTooLarge;
     g_coop_grid[slot] = cus * per;
@@ -396,14 +453,20 @@ static hipError_t coop_grid(int* G) {
     return hipSuccess;
 }
 
-hipError_t persistent_grid(bool f32, int* G) { return f32 ? coop_grid<float>(G) : coop_grid<double>(G); }
+hipError_t persistent_grid(bool f32, bool c16, int* G) {
+    hipError_t e = hipSuccess;
+    XJ_DISPATCH_VC(f32, c16, e = (coop_grid<VT, CT>(G)));
+    return e;
+}
 
-hipError_t launch_jpcg(const JpcgParams& P0, bool f32, int G, hipStream_t s) {
+hipError_t launch_jpcg(const JpcgParams& P0, bool f32, bool c16, int G, hipStream_t s) {
     JpcgParams P = P0;
     void* args[] = {&P};
-    if (f32)
-        return hipLaunchCooperativeKernel((const void*)jpcg_persistent<float>, dim3(G), dim3(kBlock), args, 0, s);
-    return hipLaunchCooperativeKernel((const void*)jpcg_persistent<double>, dim3(G), dim3(kBlock), args, 0, s);
+    hipError_t e = hipSuccess;
+    XJ_DISPATCH_VC(f32, c16,
+                   e = hipLaunchCooperativeKernel((const void*)jpcg_persistent<VT, CT>, dim3(G), dim3(kBlock), args,
+                                                  0, s));
+    return e;
 }
 
 hipError_t launch_cg_setup(const CgParams& P, hipStream_t s) {
@@ -412,11 +475,9 @@ hipError_t launch_cg_setup(const CgParams& P, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_cg_iteration(const CgParams& P, bool f32, hipStream_t s) {
-    if (f32)
-        cg_spmv<float><<<P.G1, kBlock, 0, s>>>(P);
-    else
-        cg_spmv<double><<<P.G1, kBlock, 0, s>>>(P);
+hipError_t launch_cg_iteration(const CgParams& P, bool f32, bool c16, hipStream_t s) {
+    XJ_DISPATCH_VC(f32, c16, (cg_spmv<VT, CT><<<P.ntiles, kBlock, 0, s>>>(P)));
+    if (P.split_alpha) cg_alpha<<<1, 1024, 0, s>>>(P);
     cg_update_r<<<P.G2, kBlock, 0, s>>>(P);
     cg_update_xp<<<P.G2, kBlock, 0, s>>>(P);
     return hipGetLastError();
@@ -424,12 +485,18 @@ hipError_t launch_cg_iteration(const CgParams& P, bool f32, hipStream_t s) {
 
 hipError_t launch_spmv(const CsrDev& A, const double* x, double* y, const double* b, hipStream_t s) {
     if (A.ntiles <= 0) return hipSuccess;
-    if (A.val_f32)
-        spmv_tiles<float><<<A.ntiles, kBlock, 0, s>>>(A.ntiles, A.tile_row, A.rowptr, A.col,
-                                                        static_cast<const float*>(A.val), A.dsplit, x, y, b);
-    else
-        spmv_tiles<double><<<A.ntiles, kBlock, 0, s>>>(A.ntiles, A.tile_row, A.rowptr, A.col,
-                                                         static_cast<const double*>(A.val), A.dsplit, x, y, b);
+    const void* col = A.col16 ? (const void*)A.col16 : (const void*)A.col;
+    XJ_DISPATCH_VC(A.val_f32, A.col16 != nullptr,
+                   (spmv_tiles<VT, CT><<<A.ntiles, kBlock, 0, s>>>(A.ntiles, A.tile_row, A.rowptr,
+                                                                   static_cast<const CT*>(col),
+                                                                   static_cast<const VT*>(A.val), A.dsplit, x, y,
+                                                                   b)));
+    return hipGetLastError();
+}
+
+hipError_t launch_build_col16(const CsrDev& A, int16_t* col16, int* overflow, hipStream_t s) {
+    if (A.ntiles <= 0) return hipSuccess;
+    build_col16<<<A.ntiles, kBlock, 0, s>>>(A.ntiles, A.tile_row, A.rowptr, A.col, col16, overflow);
     return hipGetLastError();
 }
 

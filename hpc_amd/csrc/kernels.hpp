@@ -125,16 +125,24 @@ __device__ __forceinline__ T ld_stream(const T* p) {
 // dsplit[i]*x[i] starts row i's sum.
 //
 // On return, lanes with *row >= 0 hold y[*row] in *y.
-template <typename VT>
+// Column index of entry k in tile row-base r0: int32 columns are stored as
+// is; int16 columns are stored as col - r0 (the compact index stream, used
+// when every tile's columns lie within 32767 of its first row).
+__device__ __forceinline__ int col_at(const int* __restrict__ col, int k, int) { return ld_stream(col + k); }
+__device__ __forceinline__ int col_at(const int16_t* __restrict__ col, int k, int r0) {
+    return r0 + (int)ld_stream(col + k);
+}
+
+template <typename VT, typename CT>
 __device__ __forceinline__ void tile_spmv(int r0, int r1, const int* __restrict__ rowptr,
-                                          const int* __restrict__ col, const VT* __restrict__ val,
+                                          const CT* __restrict__ col, const VT* __restrict__ val,
                                           const double* __restrict__ dsplit, const double* __restrict__ x,
                                           double* lds, double* red, double* y, int* row) {
     const int tid = threadIdx.x;
     const int k0 = rowptr[r0], k1 = rowptr[r1];
     if (k1 - k0 > kTileNnz) {
         double s = 0.0;
-        for (int k = k0 + tid; k < k1; k += kBlock) s += (double)ld_stream(val + k) * x[ld_stream(col + k)];
+        for (int k = k0 + tid; k < k1; k += kBlock) s += (double)ld_stream(val + k) * x[col_at(col, k, r0)];
         s = block_sum(s, red);
         if (dsplit) s = dsplit[r0] * x[r0] + s;
         *y = s;
@@ -153,7 +161,7 @@ __device__ __forceinline__ void tile_spmv(int r0, int r1, const int* __restrict_
         for (int i = 0; i < kPerThread; ++i) {
             const int e = tid + i * kBlock;
             const int k = k0 + (e < cnt ? e : 0);
-            c[i] = ld_stream(col + k);
+            c[i] = col_at(col, k, r0);
             v[i] = ld_stream(val + k);
         }
         double g[kPerThread];

@@ -10,6 +10,7 @@ struct Record;
 
 // A CSR operator resident in HBM plus its tile plan.
 //   rowptr[n+1], col[nnz]  int32
+//   col16[nnz]             optional compact column stream: col - first row of the entry's tile
 //   val[nnz]               fp64, or fp32 when val_f32
 //   dsplit[n]              optional fp64 diagonal kept outside the CSR
 //                          (the CSR then holds off-diagonal entries only)
@@ -19,6 +20,7 @@ struct CsrDev {
     int64_t nnz = 0;
     const int* rowptr = nullptr;
     const int* col = nullptr;
+    const int16_t* col16 = nullptr;
     const void* val = nullptr;
     bool val_f32 = false;
     const double* dsplit = nullptr;
@@ -29,7 +31,7 @@ struct CsrDev {
 struct JpcgParams {
     int n = 0, ntiles = 0;
     const int* rowptr = nullptr;
-    const int* col = nullptr;
+    const void* col = nullptr;           // int32, or int16 deltas when the operator is compact
     const void* val = nullptr;
     const double* dsplit = nullptr;
     const int* tile_row = nullptr;
@@ -49,6 +51,7 @@ struct JpcgParams {
 struct CgState {
     double dot, tols, rr, rz0;
     double rz[2];      // rz of iteration k lives in rz[k & 1]
+    double alpha;      // alpha of the running iteration (written by cg_alpha when the tile partials are many)
     uint32_t k;        // index of the running iteration; advanced by workgroup 0 of the SpMV kernel
     uint32_t it;       // the reference Task's iteration counter (tasks.hpp:30-47)
     int stop;          // set by workgroup 0 of the update kernel when the loop ends
@@ -57,31 +60,38 @@ struct CgState {
     int pad;
 };
 
-// Arguments of the multi-kernel loop: SpMV (G1 workgroups, tile loop), then
-// two vector kernels (G2 workgroups, grid-stride).
+// Arguments of the multi-kernel loop: SpMV (one workgroup per tile, p.Ap
+// partial per tile), optionally a one-workgroup alpha reduction, then two
+// vector kernels (G2 workgroups, grid-stride).
 struct CgParams {
-    int n = 0, ntiles = 0, G1 = 0, G2 = 0;
+    int n = 0, ntiles = 0, G2 = 0;
+    int split_alpha = 0;                 // 1: cg_alpha reduces the ntiles partials once
     const int* rowptr = nullptr;
-    const int* col = nullptr;
+    const void* col = nullptr;
     const void* val = nullptr;
     const double* dsplit = nullptr;
     const int* tile_row = nullptr;
     const double* b = nullptr;
     const double* diag = nullptr;
     double *x = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr, *J = nullptr;
-    double *partA = nullptr, *partB = nullptr, *partC = nullptr;  // G1, G2, G2 doubles
+    double *partA = nullptr, *partB = nullptr, *partC = nullptr;  // ntiles, G2, G2 doubles
     CgState* st = nullptr;
     Record* records = nullptr;
     double tol = 0.0;
     uint32_t max_rounds = 0;
 };
 
-hipError_t launch_cg_setup(const CgParams& P, hipStream_t s);
-hipError_t launch_cg_iteration(const CgParams& P, bool f32, hipStream_t s);
+// tiles above which the SpMV's p.Ap partials are reduced by a separate
+// one-workgroup kernel instead of by every workgroup of the vector kernels
+constexpr int kMaxRedundantPartials = 2048;
 
-hipError_t persistent_grid(bool f32, int* G);
-hipError_t launch_jpcg(const JpcgParams& P, bool f32, int G, hipStream_t s);
+hipError_t launch_cg_setup(const CgParams& P, hipStream_t s);
+hipError_t launch_cg_iteration(const CgParams& P, bool f32, bool c16, hipStream_t s);
+
+hipError_t persistent_grid(bool f32, bool c16, int* G);
+hipError_t launch_jpcg(const JpcgParams& P, bool f32, bool c16, int G, hipStream_t s);
 hipError_t launch_spmv(const CsrDev& A, const double* x, double* y, const double* b, hipStream_t s);
+hipError_t launch_build_col16(const CsrDev& A, int16_t* col16, int* overflow, hipStream_t s);
 int dot_blocks(int n);
 hipError_t launch_dot(int n, const double* a, const double* b, double* part, double* out, hipStream_t s);
 hipError_t launch_gather_f64(int64_t nnz, const int64_t* perm, const double* src, double* dst, hipStream_t s);

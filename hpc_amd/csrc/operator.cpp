@@ -49,6 +49,8 @@ Operator::Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t*
         rp = rp_local.data();
     }
     if (rp[0] != 0 || (int64_t)rp[n] != nnz) throw invalid("row pointers do not span [0, nnz].");
+    for (int32_t i = 0; i < n; ++i)
+        if (rp[i + 1] < rp[i]) throw invalid("row pointers must not decrease.");
     std::vector<int32_t> t = plan_tiles(n, rp);
     tiles_.ensure(t.size());
     hip_check(hipMemcpyAsync(tiles_.ptr, t.data(), sizeof(int32_t) * t.size(), hipMemcpyHostToDevice, s),
@@ -62,37 +64,63 @@ Operator::Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t*
     A_.dsplit = dsplit;
     A_.tile_row = tiles_.ptr;
     A_.ntiles = (int)t.size() - 1;
-    hip_check(persistent_grid(val_f32, &G_), "persistent kernel occupancy");
-    G1_ = std::min(A_.ntiles, 2048);
-    if (G1_ < 1) G1_ = 1;
     G2_ = std::min((n + 1023) / 1024, 2048);
     if (G2_ < 1) G2_ = 1;
-    partA_.ensure(G1_);
+    partA_.ensure(std::max(1, A_.ntiles));
     partB_.ensure(G2_);
     partC_.ensure(G2_);
     state_.ensure(1);
-    hip_check(hipEventCreateWithFlags(&ev_[0], hipEventDisableTiming), "event");
-    hip_check(hipEventCreateWithFlags(&ev_[1], hipEventDisableTiming), "event");
-    hip_check(hipHostMalloc((void**)&host_flags_, 2 * sizeof(int), hipHostMallocDefault), "pinned flags");
-    hip_check(hipHostMalloc((void**)&host_result_, 16 * sizeof(double), hipHostMallocDefault), "pinned result");
-    static_assert(sizeof(CgState) <= 16 * sizeof(double), "state fits the pinned result buffer");
-    // the vectors are only needed by solve(); allocate them now so a solve never allocates
     r_.ensure(n);
     p_.ensure(n);
     Ap_.ensure(n);
     J_.ensure(n);
-    part_.ensure((size_t)5 * G_ + 4096);
     result_.ensure(8);
     bar_.ensure(1);
     abort_.ensure(1);
+    hip_check(hipStreamCreateWithFlags(&work_, hipStreamNonBlocking), "operator stream");
+    hip_check(hipEventCreateWithFlags(&ev_[0], hipEventDisableTiming), "event");
+    hip_check(hipEventCreateWithFlags(&ev_[1], hipEventDisableTiming), "event");
+    hip_check(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming), "event");
+    hip_check(hipHostMalloc((void**)&host_flags_, 2 * sizeof(int), hipHostMallocDefault), "pinned flags");
+    hip_check(hipHostMalloc((void**)&host_result_, 16 * sizeof(double), hipHostMallocDefault), "pinned result");
+    static_assert(sizeof(CgState) <= 16 * sizeof(double), "state fits the pinned result buffer");
+    // compact int16 column stream when every tile's columns fit
+    if (nnz > 0) {
+        col16_.ensure((size_t)nnz);
+        hip_check(hipMemsetAsync(abort_.ptr, 0, sizeof(int), s), "reset flag");
+        hip_check(launch_build_col16(A_, col16_.ptr, abort_.ptr, s), "build int16 columns");
+        int overflow = 1;
+        hip_check(hipMemcpyAsync(&overflow, abort_.ptr, sizeof(int), hipMemcpyDeviceToHost, s), "int16 check");
+        hip_check(hipStreamSynchronize(s), "int16 check");
+        if (overflow) {
+            col16_.release();
+        } else {
+            has_col16_ = true;
+            A_.col16 = col16_.ptr;
+        }
+    }
+    hip_check(persistent_grid(val_f32, A_.col16 != nullptr, &G_), "persistent kernel occupancy");
+    part_.ensure((size_t)5 * G_ + 4096);
     hip_check(hipStreamSynchronize(s), "operator set-up");
 }
 
 Operator::~Operator() {
+    drop_graph();
     if (host_result_) (void)hipHostFree(host_result_);
     if (host_flags_) (void)hipHostFree(host_flags_);
     for (hipEvent_t e : ev_)
         if (e) (void)hipEventDestroy(e);
+    if (ev_join_) (void)hipEventDestroy(ev_join_);
+    if (work_) (void)hipStreamDestroy(work_);
+}
+
+void Operator::set_compact(bool on) {
+    const int16_t* want = (on && has_col16_) ? col16_.ptr : nullptr;
+    if (want == A_.col16) return;
+    A_.col16 = want;
+    drop_graph();
+    hip_check(persistent_grid(A_.val_f32, A_.col16 != nullptr, &G_), "persistent kernel occupancy");
+    part_.ensure((size_t)5 * G_ + 4096);
 }
 
 void Operator::spmv(const double* x, double* y, hipStream_t s) const {
@@ -138,15 +166,15 @@ static SolveResult finish_result(int64_t K, double rr, double dot, uint32_t max_
     return R;
 }
 
-SolveResult Operator::solve_multi(const double* diag, const double* b, double* x, uint32_t max_iter, double tol,
-                                  void* records, hipStream_t s) {
+CgParams Operator::cg_params(const double* diag, const double* b, double* x, uint32_t max_iter, double tol,
+                             void* records) const {
     CgParams P;
     P.n = A_.n;
     P.ntiles = A_.ntiles;
-    P.G1 = G1_;
     P.G2 = G2_;
+    P.split_alpha = A_.ntiles > kMaxRedundantPartials ? 1 : 0;
     P.rowptr = A_.rowptr;
-    P.col = A_.col;
+    P.col = col_stream();
     P.val = A_.val;
     P.dsplit = A_.dsplit;
     P.tile_row = A_.tile_row;
@@ -164,34 +192,82 @@ SolveResult Operator::solve_multi(const double* diag, const double* b, double* x
     P.records = static_cast<Record*>(records);
     P.tol = tol;
     P.max_rounds = max_iter + 1u;  // pcgImp.hpp:158 programs maxIter+1 rounds
-    if (records) hip_check(hipMemsetAsync(records, 0, (size_t)(max_iter + 2) * 64, s), "clear records");
-    hip_check(launch_cg_setup(P, s), "CG set-up launch");
-    // Enqueue iterations in growing chunks; after each chunk the stop flag is
-    // copied to pinned memory.  The host checks chunk c-1's flag while chunk c
-    // is queued, so the GPU never idles and at most one chunk of launches
-    // returns immediately after convergence.
+    return P;
+}
+
+void Operator::drop_graph() {
+    if (gexec_) (void)hipGraphExecDestroy(gexec_);
+    gexec_ = nullptr;
+}
+
+void Operator::build_graph(const CgParams& P) {
+    drop_graph();
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(work_, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        graph_ok_ = false;
+        return;
+    }
+    hipError_t e = hipSuccess;
+    for (int c = 0; c < kChunk && e == hipSuccess; ++c) e = launch_cg_iteration(P, A_.val_f32, A_.col16, work_);
+    hipError_t e2 = hipStreamEndCapture(work_, &g);
+    if (e == hipSuccess && e2 == hipSuccess && g) e = hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        (void)hipGetLastError();
+        gexec_ = nullptr;
+        graph_ok_ = false;
+        return;
+    }
+    gkey_ = P;
+}
+
+SolveResult Operator::solve_multi(const double* diag, const double* b, double* x, uint32_t max_iter, double tol,
+                                  void* records, hipStream_t s) {
+    const CgParams P = cg_params(diag, b, x, max_iter, tol, records);
+    // the work stream follows everything already queued on the caller's stream
+    hip_check(hipEventRecord(ev_join_, s), "event record");
+    hip_check(hipStreamWaitEvent(work_, ev_join_, 0), "stream wait");
+    if (records) hip_check(hipMemsetAsync(records, 0, (size_t)(max_iter + 2) * 64, work_), "clear records");
+    hip_check(launch_cg_setup(P, work_), "CG set-up launch");
+    // The iteration kernels read b, diag and tol only through the set-up, so a
+    // graph captured for (x, records, max_rounds, values) replays for any b.
+    if (graph_ok_ && (!gexec_ || gkey_.x != P.x || gkey_.records != P.records || gkey_.max_rounds != P.max_rounds ||
+                      gkey_.val != P.val || gkey_.col != P.col))
+        build_graph(P);
+    // Enqueue iterations in chunks; after each chunk the stop flag is copied to
+    // pinned memory.  The host checks chunk c-1's flag while chunk c is queued,
+    // so the GPU never idles and at most one chunk of launches returns at once
+    // after convergence.
     uint64_t enq = 0;
-    uint32_t chunk = 4;
     int slot = 0;
     bool pending = false;
     while (enq < P.max_rounds) {
-        const uint32_t c = (uint32_t)std::min<uint64_t>(chunk, P.max_rounds - enq);
-        for (uint32_t j = 0; j < c; ++j) hip_check(launch_cg_iteration(P, A_.val_f32, s), "CG iteration launch");
-        enq += c;
-        hip_check(hipMemcpyAsync(&host_flags_[slot], &state_.ptr->stop, sizeof(int), hipMemcpyDeviceToHost, s),
+        const uint64_t left = P.max_rounds - enq;
+        if (gexec_ && left >= (uint64_t)kChunk) {
+            hip_check(hipGraphLaunch(gexec_, work_), "graph launch");
+            enq += kChunk;
+        } else {
+            const uint32_t c = (uint32_t)std::min<uint64_t>(left, kChunk);
+            for (uint32_t j = 0; j < c; ++j)
+                hip_check(launch_cg_iteration(P, A_.val_f32, A_.col16, work_), "CG iteration launch");
+            enq += c;
+        }
+        hip_check(hipMemcpyAsync(&host_flags_[slot], &state_.ptr->stop, sizeof(int), hipMemcpyDeviceToHost, work_),
                   "read stop flag");
-        hip_check(hipEventRecord(ev_[slot], s), "event record");
+        hip_check(hipEventRecord(ev_[slot], work_), "event record");
         if (pending) {
             hip_check(hipEventSynchronize(ev_[slot ^ 1]), "event sync");
             if (host_flags_[slot ^ 1]) break;
         }
         pending = true;
         slot ^= 1;
-        chunk = std::min<uint32_t>(chunk * 2, 64);
     }
+    hip_check(hipMemcpyAsync(host_result_, state_.ptr, sizeof(CgState), hipMemcpyDeviceToHost, work_), "read state");
+    hip_check(hipEventRecord(ev_join_, work_), "event record");
+    hip_check(hipStreamWaitEvent(s, ev_join_, 0), "stream wait");
+    hip_check(hipStreamSynchronize(work_), "JPCG solve");
     CgState hs;
-    hip_check(hipMemcpyAsync(host_result_, state_.ptr, sizeof(CgState), hipMemcpyDeviceToHost, s), "read state");
-    hip_check(hipStreamSynchronize(s), "JPCG solve");
     std::memcpy(&hs, host_result_, sizeof(CgState));
     return finish_result(hs.K, hs.rr, hs.dot, max_iter);
 }
@@ -202,7 +278,7 @@ SolveResult Operator::solve_persistent(const double* diag, const double* b, doub
     P.n = A_.n;
     P.ntiles = A_.ntiles;
     P.rowptr = A_.rowptr;
-    P.col = A_.col;
+    P.col = col_stream();
     P.val = A_.val;
     P.dsplit = A_.dsplit;
     P.tile_row = A_.tile_row;
@@ -220,11 +296,10 @@ SolveResult Operator::solve_persistent(const double* diag, const double* b, doub
     P.result = result_.ptr;
     P.tol = tol;
     P.max_rounds = max_iter + 1u;  // pcgImp.hpp:158 programs maxIter+1 rounds
-    if (P.max_rounds == 0) throw invalid("maxIter too large.");
     hip_check(hipMemsetAsync(bar_.ptr, 0, sizeof(unsigned long long), s), "reset barrier");
     hip_check(hipMemsetAsync(abort_.ptr, 0, sizeof(int), s), "reset abort flag");
     if (records) hip_check(hipMemsetAsync(records, 0, (size_t)(max_iter + 2) * 64, s), "clear records");
-    hipError_t e = launch_jpcg(P, A_.val_f32, G_, s);
+    hipError_t e = launch_jpcg(P, A_.val_f32, A_.col16, G_, s);
     if (e != hipSuccess) {
         // Some runtimes refuse the cooperative launch (grid above what they will
         // make co-resident, or no cooperative support on the queue).  Clear the
@@ -232,7 +307,7 @@ SolveResult Operator::solve_persistent(const double* diag, const double* b, doub
         (void)hipGetLastError();
         if (G_ > 1) {
             G_ /= 2;
-            e = launch_jpcg(P, A_.val_f32, G_, s);
+            e = launch_jpcg(P, A_.val_f32, A_.col16, G_, s);
         }
         if (e != hipSuccess) {
             (void)hipGetLastError();

@@ -70,18 +70,27 @@ class Operator {
   public:
     // References caller device memory for rowptr/col/val/dsplit (not copied);
     // rowptr_host (n+1 entries) is used to plan tiles and may be null, in which
-    // case rowptr is copied back from the device.
+    // case rowptr is copied back from the device.  When every tile's columns
+    // lie within 32767 of its first row, an int16 copy of the column stream
+    // (col - tile's first row) is built and the SpMV kernels read it instead.
     Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t* col, const void* val, bool val_f32,
              const double* dsplit, const int32_t* rowptr_host, hipStream_t s);
+    ~Operator();
+    Operator(const Operator&) = delete;
+    Operator& operator=(const Operator&) = delete;
 
     const CsrDev& csr() const { return A_; }
     int grid() const { return G_; }
+    bool compact() const { return A_.col16 != nullptr; }
     // point the operator at another value array of the same type and length
     // (the pattern and tile plan stay); used when only A's values change
     void set_values(const void* val) {
         if (val == nullptr && A_.nnz > 0) throw invalid("Matrix is nullptr.");
         A_.val = val;
+        drop_graph();
     }
+    // use (or stop using) the int16 column stream when it exists
+    void set_compact(bool on);
 
     void spmv(const double* x, double* y, hipStream_t s) const;
     // r = b - A x; returns r.r (synchronises the stream)
@@ -99,8 +108,9 @@ class Operator {
                       hipStream_t s);
 
     // Which loop solve() runs: kPersistent is one cooperative launch with grid
-    // barriers, kMultiKernel is three launches per iteration; kAuto picks the
-    // faster measured one (the multi-kernel loop at every size measured so far).
+    // barriers, kMultiKernel is three or four launches per iteration replayed
+    // from a captured graph; kAuto picks the faster measured one (the
+    // multi-kernel loop at every size measured so far).
     enum Path { kAuto = 0, kPersistent = 1, kMultiKernel = 2 };
     void set_path(int p) { path_ = p; }
     int path() const { return path_; }
@@ -111,26 +121,35 @@ class Operator {
                                  void* records, hipStream_t s);
     SolveResult solve_multi(const double* diag, const double* b, double* x, uint32_t max_iter, double tol,
                             void* records, hipStream_t s);
+    CgParams cg_params(const double* diag, const double* b, double* x, uint32_t max_iter, double tol,
+                       void* records) const;
+    void build_graph(const CgParams& P);
+    void drop_graph();
+    const void* col_stream() const { return A_.col16 ? (const void*)A_.col16 : (const void*)A_.col; }
+
+    static constexpr int kChunk = 8;  // iterations per captured graph
 
     CsrDev A_;
+    DevBuf<int32_t> tiles_;
+    DevBuf<int16_t> col16_;
+    bool has_col16_ = false;
     int path_ = kAuto, last_path_ = 0;
-    bool coop_ok_ = true;  // false once the runtime refused the cooperative launch
-    int G1_ = 0, G2_ = 0;
+    bool coop_ok_ = true;   // false once the runtime refused the cooperative launch
+    bool graph_ok_ = true;  // false once graph capture failed; iterations are then launched directly
+    int G_ = 0, G2_ = 0;
+    DevBuf<double> r_, p_, Ap_, J_, part_, result_;
     DevBuf<double> partA_, partB_, partC_;
     DevBuf<CgState> state_;
-    hipEvent_t ev_[2] = {nullptr, nullptr};
-    int* host_flags_ = nullptr;  // pinned
-    DevBuf<int32_t> tiles_;
-    int G_ = 0;
-    DevBuf<double> r_, p_, Ap_, J_, part_, result_;
     DevBuf<unsigned long long> bar_;
     DevBuf<int> abort_;
     DevBuf<double> aux_[2];
     double* host_result_ = nullptr;  // pinned
-  public:
-    ~Operator();
-    Operator(const Operator&) = delete;
-    Operator& operator=(const Operator&) = delete;
+    int* host_flags_ = nullptr;      // pinned
+    hipStream_t work_ = nullptr;     // graphs are captured and replayed here
+    hipEvent_t ev_[2] = {nullptr, nullptr};
+    hipEvent_t ev_join_ = nullptr;
+    hipGraphExec_t gexec_ = nullptr;
+    CgParams gkey_;                  // arguments the captured graph was built with
 };
 
 }  // namespace xj

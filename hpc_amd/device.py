@@ -98,6 +98,18 @@ class CsrOperator:
         check_dev(lib().xjpcg_op_set_values(self._h, ptr(val)))
         self.val = val
 
+    def set_compact(self, on: bool) -> bool:
+        """Use the int16 column-delta stream when it exists; returns whether it is in use."""
+        act = ctypes.c_int()
+        check_dev(lib().xjpcg_op_set_compact(self._h, 1 if on else 0, ctypes.byref(act)))
+        return bool(act.value)
+
+    @property
+    def compact(self) -> bool:
+        act = ctypes.c_int()
+        check_dev(lib().xjpcg_op_set_compact(self._h, -1, ctypes.byref(act)))
+        return bool(act.value)
+
     PATH_AUTO, PATH_PERSISTENT, PATH_MULTI = 0, 1, 2
 
     def set_path(self, path: int):
@@ -151,7 +163,7 @@ class CsrOperator:
 
 
 def solve_mixed(outer: CsrOperator, inner: CsrOperator, diag: torch.Tensor, b: torch.Tensor, tol: float,
-                inner_tol: float = 1e-6, inner_max_iter: int = 20000, max_outer: int = 30,
+                inner_tol: float = 0.0, inner_max_iter: int = 20000, max_outer: int = 30,
                 x: torch.Tensor | None = None, raise_on_fail: bool = True) -> tuple[torch.Tensor, MixedInfo]:
     """fp32-valued inner JPCG inside an fp64 iterative-refinement loop."""
     if not inner.f32:

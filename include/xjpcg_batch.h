@@ -59,7 +59,9 @@ XJPCG_Status_t xjpcg_batch_solve(xjpcg_batch_t* batch, const double* d_diag, con
 
 /* Iterative refinement: `outer` (fp64-accurate values, same pattern and
  * sizes) computes the true residual r = b - A x in fp64; `inner` (typically
- * fp32 values) solves A d = r to inner_tol; x += d; repeat until every pair
+ * fp32 values) solves A d = r to max(inner_tol, 0.5 tol ||b||/||r||) for the
+ * unfinished pair furthest from tol (capped at 0.5), finished pairs masked
+ * to a zero right-hand side; x += d; repeat until every pair
  * has ||r||/||b|| < tol or max_outer steps.  d_res (fp64 [S][R], may be NULL)
  * receives the final true relative residuals. */
 XJPCG_Status_t xjpcg_batch_solve_mixed(xjpcg_batch_t* outer, xjpcg_batch_t* inner, const double* d_diag,

@@ -41,6 +41,14 @@ XJPCG_Status_t xjpcg_op_create(xjpcg_op_t** op, int32_t n, int64_t nnz, const in
                                const int32_t* d_col, const void* d_val, int val_f32, const double* d_dsplit,
                                const int32_t* h_rowptr, void* stream);
 XJPCG_Status_t xjpcg_op_destroy(xjpcg_op_t* op);
+/* Compact column stream: when every row tile's columns lie within 32767 of
+ * the tile's first row, the operator keeps an int16 copy of the column
+ * indices (col - first row of the tile; 2 bytes per entry instead of 4, in
+ * the spirit of the reference's 16-bit local indices, SPARSE_indexBits=16 in
+ * pcg/hw/Makefile:103) and its kernels read that stream.  on = 1 uses it when
+ * it exists (the default), 0 uses the int32 stream, -1 only queries;
+ * *h_active (may be NULL) tells which stream is in use. */
+XJPCG_Status_t xjpcg_op_set_compact(xjpcg_op_t* op, int on, int* h_active);
 /* Rebind the operator to another device value array of the same type and
  * length; the pattern and row tiles are kept (the device-side counterpart of
  * the reference's KEEP_NZ_LAYOUT value refresh, signature.hpp:427-469). */
@@ -87,7 +95,10 @@ XJPCG_Status_t xjpcg_sddmm(xjpcg_op_t* op, const double* d_lam, const double* d_
  *   inner: fp32-valued operator solved by the persistent JPCG (fp64 vectors,
  *          fp64 accumulation, fp64 alpha/beta dot products).
  * Loop: r = b - A x (fp64); stop when ||r||/||b|| < tol; solve inner d = r to
- * relative tolerance inner_tol (at most inner_max_iter iterations); x += d.
+ * the relative tolerance max(inner_tol, 0.5 tol ||b|| / ||r||), capped at
+ * 0.5 (at most inner_max_iter iterations); x += d.  inner_tol = 0 lets one
+ * step finish when the inner operator equals A; a larger inner_tol bounds
+ * the work per step when it only approximates A.
  * d_x receives the solution.  Counts and the final true residual are returned
  * through the h_ pointers (any may be NULL).  Returns EXECUTION_FAILED when
  * tol is not reached within max_outer refinement steps. */
