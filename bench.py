@@ -61,6 +61,30 @@ def cpu_baseline(A, budget_s=15.0):
     return out, threads_all
 
 
+def spmv_profile(args):
+    """Short run for hardware-counter passes: a plain 256 MiB copy kernel of known size, then 10 launches
+    of the fp64 SpMV on the default workload (int16 column stream when it applies)."""
+    import torch
+    from hpc_amd import device, workloads
+    dev = torch.device("cuda:0")
+    sysm = workloads.c2_stencil27(args.grid)
+    A = sysm.A
+    op = device.CsrOperator(torch.from_numpy(A.rowptr).to(dev), torch.from_numpy(A.col).to(dev),
+                            torch.from_numpy(A.val).to(dev), rowptr_host=A.rowptr)
+    src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    x = torch.ones(A.n, dtype=torch.float64, device=dev)
+    y = torch.empty_like(x)
+    for _ in range(10):
+        op.spmv(x, y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_each_way": src.numel() * 8, "spmv_algorithmic_bytes": spmv_bytes(A.n, A.nnz),
+                      "spmv_stored_bytes": (10 if op.compact else 12) * A.nnz + 4 * (A.n + 1) + 16 * A.n,
+                      "compact_columns": op.compact, "n": A.n, "nnz": A.nnz}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,11 +94,17 @@ def main():
     ap.add_argument("--spmv-repeats", type=int, default=100)
     ap.add_argument("--inner-tol", type=float, default=0.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", choices=["full", "spmv-profile"], default="full",
+                    help="spmv-profile: one 256 MiB device copy (counter calibration) then 10 SpMV launches, "
+                         "for rocprofv3 --pmc runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; the multi-GPU row split is a design "
                                    "note (DESIGN.md), run with --gpus 1"}))
         sys.exit(2)
+    if args.mode == "spmv-profile":
+        spmv_profile(args)
+        return
 
     import torch
     from hpc_amd import device, workloads

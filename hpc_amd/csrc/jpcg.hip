@@ -87,8 +87,8 @@ __global__ __launch_bounds__(kBlock) void jpcg_persistent(JpcgParams P) {
             for (int t = sw.first; t < sw.end; t += sw.step) {
                 double y;
                 int row;
-                tile_spmv<VT, CT>(P.tile_row[t], P.tile_row[t + 1], P.rowptr, col, val, P.dsplit, p, lds, red, &y,
-                                  &row);
+                tile_spmv<VT, CT>(P.tile_row[t], P.tile_row[t + 1], P.nnz, P.rowptr, col, val, P.dsplit, p, lds,
+                                  red, &y, &row);
                 if (row >= 0) {
                     Ap[row] = y;
                     pap += p[row] * y;
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(kBlock) void cg_spmv(CgParams P) {
     const double* __restrict__ p = P.p;
     double y;
     int row;
-    tile_spmv<VT, CT>(P.tile_row[t], P.tile_row[t + 1], P.rowptr, static_cast<const CT*>(P.col),
+    tile_spmv<VT, CT>(P.tile_row[t], P.tile_row[t + 1], P.nnz, P.rowptr, static_cast<const CT*>(P.col),
                       static_cast<const VT*>(P.val), P.dsplit, p, lds, red, &y, &row);
     double pap = 0.0;
     if (row >= 0) {
@@ -240,25 +240,22 @@ __global__ __launch_bounds__(kBlock) void cg_spmv(CgParams P) {
     if (threadIdx.x == 0) P.partA[t] = pap;
 }
 
-__global__ __launch_bounds__(1024) void cg_alpha(CgParams P) {
-    __shared__ double red[16];
-    CgState* st = P.st;
-    if (st->stop) return;
+// second reduction level for many tiles: workgroup b sums a contiguous slice
+// of the per-tile partials (fixed order) into partA2[b]
+__global__ __launch_bounds__(kBlock) void cg_alpha(CgParams P) {
+    __shared__ double red[kWaves];
+    if (P.st->stop) return;
+    const int b = blockIdx.x;
+    const int t0 = (int)((int64_t)P.ntiles * b / P.nA2), t1 = (int)((int64_t)P.ntiles * (b + 1) / P.nA2);
     double s = 0.0;
-    for (int i = threadIdx.x; i < P.ntiles; i += 1024) s += P.partA[i];
-    s = wave_sum(s);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = red[0];
-        for (int w = 1; w < 16; ++w) t += red[w];
-        st->alpha = st->rz[st->k & 1] / t;
-    }
+    for (int t = t0 + threadIdx.x; t < t1; t += kBlock) s += P.partA[t];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) P.partA2[b] = s;
 }
 
 __device__ __forceinline__ double cg_alpha_of(const CgParams& P, uint32_t k, double* red) {
-    if (P.split_alpha) return P.st->alpha;
-    return P.st->rz[k & 1] / grid_total(P.partA, P.ntiles, red);
+    const double pap = P.split_alpha ? grid_total(P.partA2, P.nA2, red) : grid_total(P.partA, P.ntiles, red);
+    return P.st->rz[k & 1] / pap;
 }
 
 __global__ __launch_bounds__(kBlock) void cg_update_r(CgParams P) {
@@ -329,7 +326,7 @@ __global__ __launch_bounds__(kBlock) void cg_update_xp(CgParams P) {
 // standalone kernels
 
 template <typename VT, typename CT>
-__global__ __launch_bounds__(kBlock) void spmv_tiles(int ntiles, const int* __restrict__ tile_row,
+__global__ __launch_bounds__(kBlock) void spmv_tiles(int ntiles, int nnz, const int* __restrict__ tile_row,
                                                      const int* __restrict__ rowptr, const CT* __restrict__ col,
                                                      const VT* __restrict__ val, const double* __restrict__ dsplit,
                                                      const double* __restrict__ x, double* __restrict__ y,
@@ -340,7 +337,7 @@ __global__ __launch_bounds__(kBlock) void spmv_tiles(int ntiles, const int* __re
     if (t >= ntiles) return;
     double v;
     int row;
-    tile_spmv<VT, CT>(tile_row[t], tile_row[t + 1], rowptr, col, val, dsplit, x, lds, red, &v, &row);
+    tile_spmv<VT, CT>(tile_row[t], tile_row[t + 1], nnz, rowptr, col, val, dsplit, x, lds, red, &v, &row);
     if (row >= 0) y[row] = b ? b[row] - v : v;
 }
 
@@ -401,7 +398,7 @@ __global__ __launch_bounds__(kBlock) void sddmm_tiles(int ntiles, const int* __r
     if (t >= ntiles) return;
     const int r0 = tile_row[t], r1 = tile_row[t + 1];
     const int k0 = rowptr[r0], k1 = rowptr[r1];
-    if (k1 - k0 > kTileNnz) {
+    if (k1 - k0 > kTilePlanNnz) {
         const double l = -lam[r0];
         for (int k = k0 + threadIdx.x; k < k1; k += kBlock) g[k] = l * x[col[k]];
         return;
@@ -477,7 +474,7 @@ hipError_t launch_cg_setup(const CgParams& P, hipStream_t s) {
 
 hipError_t launch_cg_iteration(const CgParams& P, bool f32, bool c16, hipStream_t s) {
     XJ_DISPATCH_VC(f32, c16, (cg_spmv<VT, CT><<<P.ntiles, kBlock, 0, s>>>(P)));
-    if (P.split_alpha) cg_alpha<<<1, 1024, 0, s>>>(P);
+    if (P.split_alpha) cg_alpha<<<P.nA2, kBlock, 0, s>>>(P);
     cg_update_r<<<P.G2, kBlock, 0, s>>>(P);
     cg_update_xp<<<P.G2, kBlock, 0, s>>>(P);
     return hipGetLastError();
@@ -487,7 +484,7 @@ hipError_t launch_spmv(const CsrDev& A, const double* x, double* y, const double
     if (A.ntiles <= 0) return hipSuccess;
     const void* col = A.col16 ? (const void*)A.col16 : (const void*)A.col;
     XJ_DISPATCH_VC(A.val_f32, A.col16 != nullptr,
-                   (spmv_tiles<VT, CT><<<A.ntiles, kBlock, 0, s>>>(A.ntiles, A.tile_row, A.rowptr,
+                   (spmv_tiles<VT, CT><<<A.ntiles, kBlock, 0, s>>>(A.ntiles, (int)A.nnz, A.tile_row, A.rowptr,
                                                                    static_cast<const CT*>(col),
                                                                    static_cast<const VT*>(A.val), A.dsplit, x, y,
                                                                    b)));

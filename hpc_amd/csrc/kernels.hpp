@@ -16,7 +16,7 @@ constexpr int kTileNnz = kTileNnzHost;     // max stored entries per short-row t
 constexpr int kTileRows = kTileRowsHost;   // max rows per tile: one lane reduces one row
 static_assert(kTileRows == kBlock, "one lane per tile row");
 static_assert(kTileNnz % kBlock == 0, "whole loads per lane");
-constexpr int kPerThread = kTileNnz / kBlock;
+
 
 // 64-byte per-iteration record, field order of the reference's CGSolverInstr
 // (pcg/hw/include/impl/cgInstr.hpp:117-132), little-endian.
@@ -113,18 +113,20 @@ __device__ __forceinline__ T ld_stream(const T* p) {
 // CSR tile SpMV.
 //
 // Rows are cut into tiles on the host: a tile is either a run of consecutive
-// rows with at most kTileRows rows and kTileNnz stored entries, or one row
-// holding more than kTileNnz entries.  For a short-row tile, lanes stream the
-// tile's values and column indices contiguously (coalesced, non-temporal),
-// gather x, and park the products in LDS; then lane t sums row t's products
-// left to right.  A long row is summed by the whole workgroup with a strided
-// loop and a fixed-order workgroup reduction.
+// rows with at most kTileRows rows and kTilePlanNnz stored entries, or one row
+// holding more than kTilePlanNnz entries.  For a short-row tile, lanes stream
+// the tile's values and column indices contiguously with 16-byte
+// non-temporal loads, gather x, and park the products in LDS; then lane t
+// sums row t's products left to right.  A long row is summed by the whole
+// workgroup with a strided loop and a fixed-order workgroup reduction.
 //
 // VT is the stored value type (double, or float for the mixed-precision
 // inner operator).  With dsplit != nullptr the matrix holds no diagonal and
-// dsplit[i]*x[i] starts row i's sum.
+// dsplit[i]*x[i] starts row i's sum.  Value and int32 column arrays must be
+// 16-byte aligned.
 //
 // On return, lanes with *row >= 0 hold y[*row] in *y.
+
 // Column index of entry k in tile row-base r0: int32 columns are stored as
 // is; int16 columns are stored as col - r0 (the compact index stream, used
 // when every tile's columns lie within 32767 of its first row).
@@ -132,15 +134,31 @@ __device__ __forceinline__ int col_at(const int* __restrict__ col, int k, int) {
 __device__ __forceinline__ int col_at(const int16_t* __restrict__ col, int k, int r0) {
     return r0 + (int)ld_stream(col + k);
 }
+__device__ __forceinline__ int col_of(int c, int) { return c; }
+__device__ __forceinline__ int col_of(int16_t c, int r0) { return r0 + (int)c; }
+
+// E consecutive elements of T as one non-temporal load of E*sizeof(T) bytes
+// (p must be aligned to that size).
+template <int BYTES> struct LoadWord;
+template <> struct LoadWord<4> { typedef uint32_t type; };
+template <> struct LoadWord<8> { typedef uint64_t type; };
+template <> struct LoadWord<16> { typedef unsigned int type __attribute__((ext_vector_type(4))); };
+
+template <typename T, int E>
+__device__ __forceinline__ void load_run(const T* __restrict__ p, T (&out)[E]) {
+    typedef typename LoadWord<E * sizeof(T)>::type W;
+    const W w = __builtin_nontemporal_load(reinterpret_cast<const W*>(p));
+    __builtin_memcpy(out, &w, sizeof(W));
+}
 
 template <typename VT, typename CT>
-__device__ __forceinline__ void tile_spmv(int r0, int r1, const int* __restrict__ rowptr,
+__device__ __forceinline__ void tile_spmv(int r0, int r1, int nnz, const int* __restrict__ rowptr,
                                           const CT* __restrict__ col, const VT* __restrict__ val,
                                           const double* __restrict__ dsplit, const double* __restrict__ x,
                                           double* lds, double* red, double* y, int* row) {
     const int tid = threadIdx.x;
     const int k0 = rowptr[r0], k1 = rowptr[r1];
-    if (k1 - k0 > kTileNnz) {
+    if (k1 - k0 > kTilePlanNnz) {
         double s = 0.0;
         for (int k = k0 + tid; k < k1; k += kBlock) s += (double)ld_stream(val + k) * x[col_at(col, k, r0)];
         s = block_sum(s, red);
@@ -149,28 +167,56 @@ __device__ __forceinline__ void tile_spmv(int r0, int r1, const int* __restrict_
         *row = (tid == 0) ? r0 : -1;
         return;
     }
-    const int cnt = k1 - k0;
-    if (cnt > 0) {
-        // Branch-free streaming: lanes past the tile's end re-read entry k0
-        // (a valid address) and their products are dropped, so all loads of a
-        // lane issue back to back and the gathers overlap instead of waiting
-        // one by one.
-        int c[kPerThread];
-        VT v[kPerThread];
+    if (k1 > k0) {
+        // Each lane loads runs of E = 16/sizeof(VT) consecutive entries (one
+        // 16-byte value load, one matching column load), kGroups runs spaced a
+        // workgroup-span apart, so every wave-wide load is one contiguous
+        // block.  The range starts at k0 rounded down to a multiple of E (the
+        // plan leaves room for the extra entries); entries outside [k0, k1)
+        // are dropped.  Every E-aligned run that meets [k0, k1) lies inside
+        // the arrays unless the tile ends in a final partial run of the whole
+        // array (nnz not a multiple of E); only that tile loads entry by
+        // entry.  Lanes with nothing to load re-read the tile's first run.
+        // All loads of a lane issue before its gathers, and all gathers
+        // before the products, so their latencies overlap.
+        constexpr int E = 16 / (int)sizeof(VT);
+        constexpr int kGroups = kTileNnz / (kBlock * E);
+        const int k0a = k0 & ~(E - 1);
+        VT v[kGroups][E];
+        CT c[kGroups][E];
+        if (((k1 + E - 1) & ~(E - 1)) <= nnz) {
 #pragma unroll
-        for (int i = 0; i < kPerThread; ++i) {
-            const int e = tid + i * kBlock;
-            const int k = k0 + (e < cnt ? e : 0);
-            c[i] = col_at(col, k, r0);
-            v[i] = ld_stream(val + k);
+            for (int g = 0; g < kGroups; ++g) {
+                int base = k0a + (g * kBlock + tid) * E;
+                if (base >= k1) base = k0a;
+                load_run<VT, E>(val + base, v[g]);
+                load_run<CT, E>(col + base, c[g]);
+            }
+        } else {
+#pragma unroll
+            for (int g = 0; g < kGroups; ++g) {
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const int kk = k0a + (g * kBlock + tid) * E + e;
+                    const int k = kk >= k0 && kk < k1 ? kk : k0;
+                    v[g][e] = ld_stream(val + k);
+                    c[g][e] = ld_stream(col + k);
+                }
+            }
         }
-        double g[kPerThread];
+        double gx[kGroups][E];
 #pragma unroll
-        for (int i = 0; i < kPerThread; ++i) g[i] = x[c[i]];
+        for (int g = 0; g < kGroups; ++g)
 #pragma unroll
-        for (int i = 0; i < kPerThread; ++i) {
-            const int e = tid + i * kBlock;
-            if (e < cnt) lds[e] = (double)v[i] * g[i];
+            for (int e = 0; e < E; ++e) gx[g][e] = x[col_of(c[g][e], r0)];
+#pragma unroll
+        for (int g = 0; g < kGroups; ++g) {
+            const int base = k0a + (g * kBlock + tid) * E;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int k = base + e;
+                if (k >= k0 && k < k1) lds[k - k0] = (double)v[g][e] * gx[g][e];
+            }
         }
     }
     __syncthreads();

@@ -29,7 +29,7 @@ struct CsrDev {
 };
 
 struct JpcgParams {
-    int n = 0, ntiles = 0;
+    int n = 0, ntiles = 0, nnz = 0;
     const int* rowptr = nullptr;
     const void* col = nullptr;           // int32, or int16 deltas when the operator is compact
     const void* val = nullptr;
@@ -64,8 +64,10 @@ struct CgState {
 // partial per tile), optionally a one-workgroup alpha reduction, then two
 // vector kernels (G2 workgroups, grid-stride).
 struct CgParams {
-    int n = 0, ntiles = 0, G2 = 0;
-    int split_alpha = 0;                 // 1: cg_alpha reduces the ntiles partials once
+    int n = 0, ntiles = 0, nnz = 0, G2 = 0;
+    int split_alpha = 0;                 // 1: cg_alpha first folds the ntiles partials into nA2
+    int nA2 = 0;
+    double* partA2 = nullptr;            // nA2 doubles
     const int* rowptr = nullptr;
     const void* col = nullptr;
     const void* val = nullptr;
@@ -81,9 +83,11 @@ struct CgParams {
     uint32_t max_rounds = 0;
 };
 
-// tiles above which the SpMV's p.Ap partials are reduced by a separate
-// one-workgroup kernel instead of by every workgroup of the vector kernels
+// tiles above which the SpMV's p.Ap partials are first folded by cg_alpha
+// into kAlphaBlocks partials instead of being summed by every workgroup of
+// the vector kernels
 constexpr int kMaxRedundantPartials = 2048;
+constexpr int kAlphaBlocks = 128;
 
 hipError_t launch_cg_setup(const CgParams& P, hipStream_t s);
 hipError_t launch_cg_iteration(const CgParams& P, bool f32, bool c16, hipStream_t s);

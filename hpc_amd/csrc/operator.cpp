@@ -23,10 +23,10 @@ std::vector<int32_t> plan_tiles(int32_t n, const int32_t* rp) {
     int32_t r = 0;
     while (r < n) {
         const int32_t start = r;
-        if (rp[r + 1] - rp[r] > kTileNnzHost) {
+        if (rp[r + 1] - rp[r] > kTilePlanNnz) {
             r += 1;  // one long row
         } else {
-            while (r < n && r - start < kTileRowsHost && rp[r + 1] - rp[start] <= kTileNnzHost) ++r;
+            while (r < n && r - start < kTileRowsHost && rp[r + 1] - rp[start] <= kTilePlanNnz) ++r;
         }
         t.push_back(r);
     }
@@ -39,6 +39,8 @@ Operator::Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t*
     if (nnz < 0 || nnz >= (int64_t)INT32_MAX) throw invalid("number of stored entries must be in [0, 2^31-1).");
     if (rowptr == nullptr || (nnz > 0 && (col == nullptr || val == nullptr)))
         throw invalid("Matrix is nullptr.");
+    if ((reinterpret_cast<uintptr_t>(col) | reinterpret_cast<uintptr_t>(val)) & 15u)
+        throw invalid("column and value arrays must be 16-byte aligned.");
     std::vector<int32_t> rp_local;
     const int32_t* rp = rowptr_host;
     if (rp == nullptr) {
@@ -67,6 +69,7 @@ Operator::Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t*
     G2_ = std::min((n + 1023) / 1024, 2048);
     if (G2_ < 1) G2_ = 1;
     partA_.ensure(std::max(1, A_.ntiles));
+    partA2_.ensure(kAlphaBlocks);
     partB_.ensure(G2_);
     partC_.ensure(G2_);
     state_.ensure(1);
@@ -171,8 +174,11 @@ CgParams Operator::cg_params(const double* diag, const double* b, double* x, uin
     CgParams P;
     P.n = A_.n;
     P.ntiles = A_.ntiles;
+    P.nnz = (int)A_.nnz;
     P.G2 = G2_;
     P.split_alpha = A_.ntiles > kMaxRedundantPartials ? 1 : 0;
+    P.nA2 = kAlphaBlocks;
+    P.partA2 = partA2_.ptr;
     P.rowptr = A_.rowptr;
     P.col = col_stream();
     P.val = A_.val;
@@ -277,6 +283,7 @@ SolveResult Operator::solve_persistent(const double* diag, const double* b, doub
     JpcgParams P;
     P.n = A_.n;
     P.ntiles = A_.ntiles;
+    P.nnz = (int)A_.nnz;
     P.rowptr = A_.rowptr;
     P.col = col_stream();
     P.val = A_.val;

@@ -86,6 +86,7 @@ class Operator {
     // (the pattern and tile plan stay); used when only A's values change
     void set_values(const void* val) {
         if (val == nullptr && A_.nnz > 0) throw invalid("Matrix is nullptr.");
+        if (reinterpret_cast<uintptr_t>(val) & 15u) throw invalid("value array must be 16-byte aligned.");
         A_.val = val;
         drop_graph();
     }
@@ -138,7 +139,7 @@ class Operator {
     bool graph_ok_ = true;  // false once graph capture failed; iterations are then launched directly
     int G_ = 0, G2_ = 0;
     DevBuf<double> r_, p_, Ap_, J_, part_, result_;
-    DevBuf<double> partA_, partB_, partC_;
+    DevBuf<double> partA_, partA2_, partB_, partC_;
     DevBuf<CgState> state_;
     DevBuf<unsigned long long> bar_;
     DevBuf<int> abort_;
