@@ -204,11 +204,17 @@ __device__ __forceinline__ void tile_spmv(int r0, int r1, int nnz, const int* __
                 }
             }
         }
+        // Entries outside [k0, k1) belong to neighbouring tiles: their int16
+        // deltas are relative to another tile's first row, so decoding them
+        // here could index past x.  They gather x[r0] instead (and are dropped).
         double gx[kGroups][E];
 #pragma unroll
         for (int g = 0; g < kGroups; ++g)
 #pragma unroll
-            for (int e = 0; e < E; ++e) gx[g][e] = x[col_of(c[g][e], r0)];
+            for (int e = 0; e < E; ++e) {
+                const int k = k0a + (g * kBlock + tid) * E + e;
+                gx[g][e] = x[(k >= k0 && k < k1) ? col_of(c[g][e], r0) : r0];
+            }
 #pragma unroll
         for (int g = 0; g < kGroups; ++g) {
             const int base = k0a + (g * kBlock + tid) * E;
