@@ -61,6 +61,16 @@ def cpu_baseline(A, budget_s=15.0):
     return out, threads_all
 
 
+def measured_traffic(grid, compact):
+    """HBM bytes per SpMV launch from the committed counter summary (only for the workload it measured)."""
+    import re
+    path = os.path.join(ROOT, "profiles", "spmv_tiles_stencil27_160.md")
+    if grid != 160 or not compact or not os.path.exists(path):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", open(path).read(), re.M)
+    return int(m.group(1)) if m else None
+
+
 def spmv_profile(args):
     """Short run for hardware-counter passes: a plain 256 MiB copy kernel of known size, then 10 launches
     of the fp64 SpMV on the default workload (int16 column stream when it applies)."""
@@ -247,8 +257,12 @@ def main():
             "speedup_mixed_vs_fp64": round(fp64_ms / mixed_ms, 3),
         },
         "roofline": {"bound": "hbm", "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "spmv_tiles<double>", "algorithmic_bytes_per_launch": nb},
+                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(args.grid, op.compact),
+                     "kernel": "spmv_tiles<double, %s columns>" % ("int16-delta" if op.compact else "int32"),
+                     "algorithmic_bytes_per_launch": nb,
+                     "traffic_source": "profiles/spmv_tiles_stencil27_160.md (rocprofv3 FETCH_SIZE/WRITE_SIZE, "
+                                       "copy-calibrated)"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
