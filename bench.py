@@ -95,6 +95,223 @@ def spmv_profile(args):
                       "compact_columns": op.compact, "n": A.n, "nnz": A.nnz}), flush=True)
 
 
+def _events(stream):
+    import torch
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    return e0, e1
+
+
+def _timed(fn, steps, warmup, stream):
+    """Runs fn warmup times untimed, then steps times between HIP events on `stream`; (ms list, last result)."""
+    import torch
+    out = None
+    for _ in range(warmup):
+        out = fn()
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(steps):
+        e0, e1 = _events(stream)
+        out = fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    return ms, out
+
+
+def _spmv_rate(op, n, stream, repeats, nbytes):
+    import torch
+    x = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        op.spmv(x, y)
+    e0, e1 = _events(stream)
+    for _ in range(repeats):
+        op.spmv(x, y)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / repeats
+    return {"us": round(us, 2), "GBs": round(nbytes / (us * 1e-6) / 1e9, 1), "bytes_per_launch": nbytes}
+
+
+def bench_graph(args):
+    """Graph Laplacian + 1e-2 I (lognormal degrees mu 2.5 sigma 1 capped at 2000, weights U[0.1,1] in fp32,
+    fp64 diagonal): fp64 JPCG vs the fp32-valued refined solve to 1e-10, both checked against the oracle
+    (the same JPCG on host cores)."""
+    import torch
+    from hpc_amd import device, workloads
+    from oracle import oracle
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    sysm = workloads.c3_graph(args.graph_n)
+    A = sysm.A
+    deg = np.diff(A.rowptr)
+    log(f"graph n={A.n} nnz={A.nnz} max row {deg.max()} generated in {time.time() - t0:.1f}s")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    op = device.CsrOperator(t(A.rowptr), t(A.col), t(A.val), rowptr_host=A.rowptr)
+    srp, scol, sv32, sdiag, exact = device.split_fp32(A.rowptr, A.col, A.val)
+    inner = device.CsrOperator(t(srp), t(scol), t(sv32), dsplit=t(sdiag), rowptr_host=srp)
+    # residual operator: the split one when it equals A (fewer bytes), else the fp64 one; a separate
+    # object from the inner operator so their work buffers stay apart
+    outer = device.CsrOperator(t(srp), t(scol), t(sv32), dsplit=t(sdiag), rowptr_host=srp) if exact else op
+    diag, b = t(A.diag), t(sysm.b)
+    stream = torch.cuda.current_stream()
+    # SpMV in both layouts (row tiles, panel-swept copy); the solves use each operator's creation-time choice
+    chosen = {"fp64": op.layout, "fp32_values": inner.layout}
+    sp64, sp32 = {}, {}
+    for lay, name in ((1, "tiles"), (2, "panels")):
+        op.set_layout(lay)
+        inner.set_layout(lay)
+        sp64[name] = _spmv_rate(op, A.n, stream, args.spmv_repeats, spmv_bytes(A.n, A.nnz))
+        sp32[name] = _spmv_rate(inner, A.n, stream, args.spmv_repeats, 8 * inner.nnz + 4 * (A.n + 1) + 24 * A.n)
+    for o in (op, inner, outer):
+        o.set_layout(0)
+    log(f"SpMV fp64 {sp64}  fp32-values {sp32}  layouts at creation {chosen} op={op.info()} compact={op.compact}")
+    x = torch.empty_like(b)
+    ms64, (x, info) = _timed(lambda: op.solve(diag, b, sysm.max_iter, sysm.tol, x=x), args.steps, args.warmup, stream)
+    xm = torch.empty_like(b)
+    msm, (xm, minfo) = _timed(lambda: device.solve_mixed(outer, inner, diag, b, sysm.tol, inner_tol=args.inner_tol,
+                                                         x=xm), args.steps, args.warmup, stream)
+    bb = float(torch.dot(b, b))
+    true64 = (op.residual(b, x)[1] / bb) ** 0.5
+    truem = (op.residual(b, xm)[1] / bb) ** 0.5
+    log(f"fp64 {np.mean(ms64):.2f} ms ({info.executed} it, true res {true64:.2e}); mixed {np.mean(msm):.2f} ms "
+        f"(outer {minfo.outer}, inner {minfo.inner_total}, true res {truem:.2e})")
+    th = min(os.cpu_count() or 1, 16)
+    log(f"oracle JPCG on {th} host threads ...")
+    t0 = time.perf_counter()
+    ref = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, sysm.b, sysm.max_iter, sysm.tol, threads=th)
+    cpu_s = time.perf_counter() - t0
+    xr = ref["x"]
+    nr = np.linalg.norm(xr)
+    e64 = float(np.linalg.norm(x.cpu().numpy() - xr) / nr)
+    em = float(np.linalg.norm(xm.cpu().numpy() - xr) / nr)
+    out = {"metric": "graph-Laplacian JPCG time-to-1e-10: mixed-refined vs fp64, 1 GPU",
+           "workload": f"graph Laplacian + 1e-2 I, n={A.n}, nnz={A.nnz}, degrees lognormal(2.5, 1.0) cap 2000, "
+                       "weights U[0.1,1] (fp32-exact), b U[-1,1] seed 2, tol 1e-10",
+           "steps": args.steps, "warmup": args.warmup, "split_exact": exact,
+           "spmv": {"fp64": sp64, "fp32_values": sp32, "compact_columns": op.compact,
+                    "layout_used": {k: ("panels" if v == 2 else "tiles") for k, v in chosen.items()}},
+           "fp64": {"ms": round(float(np.mean(ms64)), 3), "iterations": info.executed, "true_res": true64,
+                    "rel_err_vs_oracle": e64},
+           "mixed": {"ms": round(float(np.mean(msm)), 3), "outer": minfo.outer, "inner_total": minfo.inner_total,
+                     "true_res": truem, "rel_err_vs_oracle": em},
+           "speedup_mixed_vs_fp64": round(float(np.mean(ms64) / np.mean(msm)), 3),
+           "oracle": {"iterations": ref["executed"], "res": ref["res"], "seconds": round(cpu_s, 2), "threads": th}}
+    print(json.dumps(out), flush=True)
+
+
+def bench_batched(args):
+    """512 anisotropic 2D/3D diffusion systems (n log-uniform in [5k, 400k]) x 16 right-hand sides, tol 1e-8:
+    batched fp64 JPCG vs the fp32-valued refined batch, with per-pair checks against the oracle on a sample."""
+    import torch
+    from hpc_amd import batched, workloads
+    from oracle import oracle
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    mats, rhs = workloads.c4_systems(args.batch_count, 5000, 400_000, args.nrhs, seed=3)
+    asm64 = batched.assemble(mats)
+    asm32 = batched.assemble(mats, split_fp32=True)
+    N, R = int(asm64["sys_row"][-1]), args.nrhs
+    log(f"{len(mats)} systems, N={N}, nnz={asm64['col'].size}, R={R}, generated in {time.time() - t0:.1f}s")
+    bh = np.concatenate(rhs)
+    del rhs
+    b = torch.from_numpy(bh).to(dev)
+    diag = torch.from_numpy(asm64["diag"]).to(dev)
+    B64 = batched.Batch.from_host(asm64, R, dev)
+    outer = batched.Batch.from_host(asm32, R, dev)
+    inner = batched.Batch.from_host(asm32, R, dev)
+    stream = torch.cuda.current_stream()
+    t0 = time.time()
+    B64.configure(tune=True)
+    inner.configure(tune=True)
+    log(f"autotuned in {time.time() - t0:.1f}s: fp64 {B64.info()} fp32 {inner.info()}")
+    tol, maxit = 1e-8, 100000
+    x = torch.empty_like(b)
+    ms64, (x, info) = _timed(lambda: B64.solve(diag, b, maxit, tol, x=x), args.steps, args.warmup, stream)
+    its = info.iterations.cpu().numpy()
+    log(f"fp64 batch: {np.mean(ms64):.1f} ms, {info.launched} launched iterations, pair iterations "
+        f"{its.min()}..{its.max()}, max res {float(info.residual.max()):.2e}")
+    xm = torch.empty_like(b)
+    msm, (xm, minfo, steps) = _timed(lambda: batched.solve_mixed(outer, inner, diag, b, tol, inner_tol=args.inner_tol,
+                                                                 x=xm), args.steps, args.warmup, stream)
+    log(f"mixed batch: {np.mean(msm):.1f} ms, outer {steps}, launched {minfo.launched}, max true res "
+        f"{float(minfo.residual.max()):.2e}")
+    sizes = np.diff(asm64["sys_row"])
+    order = np.argsort(sizes)
+    picks = [int(order[0]), int(order[len(order) // 4]), int(order[len(order) // 2])]
+    th = min(os.cpu_count() or 1, 16)
+    xh, xmh = x.cpu().numpy(), xm.cpu().numpy()
+    checks = []
+    for s in picks:
+        r0, r1 = int(asm64["sys_row"][s]), int(asm64["sys_row"][s + 1])
+        A = mats[s]
+        for j in (0, R - 1):
+            ref = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, bh[r0:r1, j].copy(), maxit, tol, threads=th)
+            nr = np.linalg.norm(ref["x"])
+            checks.append({"system": s, "n": A.n, "rhs": j, "oracle_iterations": ref["executed"],
+                           "iterations": int(its[s, j]),
+                           "fp64_rel_err": float(np.linalg.norm(xh[r0:r1, j] - ref["x"]) / nr),
+                           "mixed_rel_err": float(np.linalg.norm(xmh[r0:r1, j] - ref["x"]) / nr)})
+            log(f"check {checks[-1]}")
+    out = {"metric": "batched JPCG time-to-1e-8 (all pairs): mixed-refined vs fp64, 1 GPU",
+           "workload": f"{len(mats)} anisotropic 2D5/3D7 diffusion systems, n log-uniform [5k,400k] (N={N}, "
+                       f"nnz={asm64['col'].size}), {R} rhs U[-1,1] seed 3, tol 1e-8",
+           "steps": args.steps, "warmup": args.warmup,
+           "fp64": {"ms": round(float(np.mean(ms64)), 2), "launched_iterations": info.launched,
+                    "pair_iterations_max": int(its.max()), "pair_iterations_sum": int(its.sum()),
+                    "max_res": float(info.residual.max()), "config": B64.info()},
+           "mixed": {"ms": round(float(np.mean(msm)), 2), "outer": steps, "launched_iterations": minfo.launched,
+                     "inner_tol": args.inner_tol, "max_true_res": float(minfo.residual.max()),
+                     "config": inner.info()},
+           "speedup_mixed_vs_fp64": round(float(np.mean(ms64) / np.mean(msm)), 3),
+           "checks": checks}
+    print(json.dumps(out), flush=True)
+
+
+def bench_autograd(args):
+    """Differentiable solve on 2D variable-coefficient diffusion 512^2: forward JPCG, then backward
+    (adjoint JPCG + SDDMM) for L = 0.5 ||x - x_target||^2, fp64."""
+    import torch
+    from hpc_amd import autograd, workloads
+    dev = torch.device("cuda:0")
+    A, bh, xt = workloads.c5_diffusion()
+    pat = autograd.SparsePattern(A.rowptr, A.col, dev)
+    base = torch.from_numpy(A.val.copy()).to(dev)
+    b0 = torch.from_numpy(bh).to(dev)
+    xt_d = torch.from_numpy(xt).to(dev)
+    stream = torch.cuda.current_stream()
+    tol = 1e-12
+
+    def fwd_bwd():
+        v = base.clone().requires_grad_(True)
+        bt = b0.clone().requires_grad_(True)
+        e0, e1 = _events(stream)
+        x = autograd.jpcg_solve(pat, v, bt, tol=tol, max_iter=200000)
+        e1.record(stream)
+        loss = 0.5 * ((x - xt_d) ** 2).sum()
+        e2, e3 = _events(stream)
+        loss.backward()
+        e3.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1), e2.elapsed_time(e3), v.grad, bt.grad
+
+    for _ in range(args.warmup):
+        fwd_bwd()
+    f, bw = [], []
+    for _ in range(args.steps):
+        a, c, gv, gb = fwd_bwd()
+        f.append(a)
+        bw.append(c)
+    out = {"metric": "differentiable JPCG forward + backward (adjoint JPCG + SDDMM), 1 GPU",
+           "workload": f"2D variable-coefficient diffusion 512^2 (n={A.n}, nnz={A.nnz}), exp(GRF) conductivities "
+                       f"(length 16, variance 1, seed 4), tol {tol}",
+           "steps": args.steps, "warmup": args.warmup,
+           "forward_ms": round(float(np.mean(f)), 3), "backward_ms": round(float(np.mean(bw)), 3),
+           "grad_values_norm": float(gv.norm()), "grad_b_norm": float(gb.norm())}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,9 +321,13 @@ def main():
     ap.add_argument("--spmv-repeats", type=int, default=100)
     ap.add_argument("--inner-tol", type=float, default=0.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", choices=["full", "spmv-profile"], default="full",
+    ap.add_argument("--mode", choices=["full", "spmv-profile", "graph", "batched", "autograd"], default="full",
                     help="spmv-profile: one 256 MiB device copy (counter calibration) then 10 SpMV launches, "
-                         "for rocprofv3 --pmc runs")
+                         "for rocprofv3 --pmc runs; graph / batched / autograd: the graph-Laplacian, batched "
+                         "and differentiable workloads, one JSON line each")
+    ap.add_argument("--graph-n", type=int, default=4_000_000)
+    ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--nrhs", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; the multi-GPU row split is a design "
@@ -114,6 +335,12 @@ def main():
         sys.exit(2)
     if args.mode == "spmv-profile":
         spmv_profile(args)
+        return
+    if args.mode in ("graph", "batched", "autograd"):
+        import torch
+        assert torch.cuda.is_available(), "bench.py needs a GPU"
+        torch.cuda.set_device(0)
+        {"graph": bench_graph, "batched": bench_batched, "autograd": bench_autograd}[args.mode](args)
         return
 
     import torch

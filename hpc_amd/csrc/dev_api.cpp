@@ -74,6 +74,39 @@ XJPCG_Status_t xjpcg_op_set_values(xjpcg_op_t* op, const void* d_val) {
     return guarded([&] { impl_of(op).set_values(d_val); });
 }
 
+XJPCG_Status_t xjpcg_panel_dims(int32_t n, int32_t* block_rows, int32_t* nb, int32_t* npanels) {
+    return guarded([&] {
+        if (n <= 0) throw xj::invalid("Wrong dimension size.");
+        const int br = xj::panel_block_rows(n);
+        if (block_rows) *block_rows = br;
+        if (nb) *nb = (int32_t)(((int64_t)n + br - 1) / br);
+        if (npanels) *npanels = (int32_t)(((int64_t)n + 65535) / 65536);
+    });
+}
+
+XJPCG_Status_t xjpcg_panel_plan(int32_t n, const int32_t* h_rowptr, const int32_t* h_col, int32_t* seg,
+                                uint16_t* off, uint16_t* lrow, int64_t* perm, int* h_ordered) {
+    return guarded([&] {
+        if (n <= 0) throw xj::invalid("Wrong dimension size.");
+        if (!h_rowptr || !seg || (h_rowptr[n] > 0 && (!h_col || !off || !lrow || !perm)))
+            throw xj::invalid("Matrix is nullptr.");
+        const xj::PanelPlan pl = xj::plan_panels(n, h_rowptr, h_col, xj::panel_block_rows(n));
+        std::copy(pl.seg.begin(), pl.seg.end(), seg);
+        std::copy(pl.off.begin(), pl.off.end(), off);
+        std::copy(pl.lrow.begin(), pl.lrow.end(), lrow);
+        std::copy(pl.perm.begin(), pl.perm.end(), perm);
+        if (h_ordered) *h_ordered = pl.ordered ? 1 : 0;
+    });
+}
+
+XJPCG_Status_t xjpcg_op_set_layout(xjpcg_op_t* op, int layout, void* stream, int* h_active) {
+    return guarded([&] {
+        xj::Operator& o = impl_of(op);
+        if (layout >= 0) o.set_layout(layout, as_stream(stream));
+        if (h_active) *h_active = o.layout();
+    });
+}
+
 XJPCG_Status_t xjpcg_op_info(const xjpcg_op_t* op, int32_t* n, int64_t* nnz, int32_t* ntiles, int32_t* grid) {
     return guarded([&] {
         const xj::Operator& o = impl_of(const_cast<xjpcg_op_t*>(op));

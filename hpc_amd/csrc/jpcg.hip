@@ -10,6 +10,8 @@
 //  dot_partials / sum_partials   fixed-order two-pass dot product
 //  gather_values     KEEP_NZ_LAYOUT value refresh through the stored permutation
 //  sddmm_tiles       g_k = -lam[row_k] * x[col_k] on the sparsity pattern
+//  spmv_panels / cg_spmv_panels   SpMV on the panel-swept layout (operators
+//                    whose columns scatter over x), standalone and in the CG loop
 //
 // Kernels that read the matrix are instantiated for fp64 / fp32 values (VT)
 // and int32 / int16-delta columns (CT).
@@ -341,6 +343,115 @@ __global__ __launch_bounds__(kBlock) void spmv_tiles(int ntiles, int nnz, const 
     if (row >= 0) y[row] = b ? b[row] - v : v;
 }
 
+// --------------------------------------------------------------------------
+// Panel-swept SpMV.
+//
+// For operators whose columns scatter over the whole of x (random graphs),
+// every gather of the tile kernel misses the XCD's L2 and costs a full line
+// fetch from the MALL or HBM.  In the panel layout, row block b keeps its
+// rows' running sums in LDS and walks the column panels (kPanelCols columns
+// each) in order; all workgroups walk them in the same order, so the x
+// entries being gathered at any moment lie in one 512 KB window that stays in
+// L2.  Block b's entries of panel p are segment [seg[b*P+p], seg[b*P+p+1]),
+// row-major with ascending columns.  A chunk of up to kPanelChunk entries is
+// loaded (value, uint16 column offset, uint16 row offset), the products are
+// staged in LDS, and the lane holding the first entry of each row's run in
+// the chunk adds that run to the row's sum left to right.  A row's runs in
+// later chunks and panels hold larger columns and are added after it, so
+// every row is summed left to right from dsplit[i]*x[i] (or 0), exactly as
+// the tile kernel sums a short row.
+constexpr int kPanelCols = kPanelColsHost;
+constexpr int kPanelBlock = kPanelBlockHost;
+constexpr int kPanelChunk = kPanelChunkHost;
+static_assert(kPanelChunk % kPanelBlock == 0, "whole chunk entries per lane");
+
+// acc: LDS of panel_lds_bytes(A.block_rows) bytes; on return acc[i] holds
+// y[b*block_rows + i] for the block's rows (after a __syncthreads).
+template <typename VT>
+__device__ __forceinline__ void panel_rows(const PanelDev& A, int n, int b, const double* __restrict__ dsplit,
+                                           const double* __restrict__ x, double* acc) {
+    constexpr int J = kPanelChunk / kPanelBlock;
+    double* prod = acc + A.block_rows;
+    uint16_t* lr = reinterpret_cast<uint16_t*>(prod + kPanelChunk);
+    const int tid = threadIdx.x;
+    const int row0 = b * A.block_rows;
+    const int rows = min(A.block_rows, n - row0);
+    for (int i = tid; i < rows; i += kPanelBlock) acc[i] = dsplit ? dsplit[row0 + i] * x[row0 + i] : 0.0;
+    __syncthreads();
+    const int* seg = A.seg + (int64_t)b * A.P;
+    const VT* __restrict__ val = static_cast<const VT*>(A.val);
+    const uint16_t* __restrict__ off = A.off;
+    const uint16_t* __restrict__ lrow = A.lrow;
+    for (int p = 0; p < A.P; ++p) {
+        const int s1 = seg[p + 1];
+        const double* __restrict__ xp = x + (int64_t)p * kPanelCols;
+        for (int c0 = seg[p]; c0 < s1; c0 += kPanelChunk) {
+            const int cn = min(kPanelChunk, s1 - c0);
+            // lanes past the chunk's end re-read its first entry (in bounds)
+            VT v[J];
+            int o[J];
+#pragma unroll
+            for (int j = 0; j < J; ++j) {
+                const int k = tid + j * kPanelBlock;
+                const int e = c0 + (k < cn ? k : 0);
+                v[j] = ld_stream(val + e);
+                o[j] = off[e];
+                lr[k] = k < cn ? lrow[e] : (uint16_t)0xFFFF;
+            }
+#pragma unroll
+            for (int j = 0; j < J; ++j) prod[tid + j * kPanelBlock] = (double)v[j] * xp[o[j]];
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < J; ++j) {
+                const int k = tid + j * kPanelBlock;
+                if (k < cn) {
+                    const int r = lr[k];
+                    if (k == 0 || lr[k - 1] != r) {
+                        double a = acc[r];
+                        int q = k;
+                        while (q < cn && lr[q] == r) a += prod[q++];
+                        acc[r] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <typename VT>
+__global__ __launch_bounds__(kPanelBlock) void spmv_panels(PanelDev A, int n, const double* __restrict__ dsplit,
+                                                           const double* __restrict__ x, double* __restrict__ y,
+                                                           const double* __restrict__ b) {
+    extern __shared__ double plds[];
+    const int blk = blockIdx.x;
+    panel_rows<VT>(A, n, blk, dsplit, x, plds);
+    const int row0 = blk * A.block_rows, rows = min(A.block_rows, n - row0);
+    for (int i = threadIdx.x; i < rows; i += kPanelBlock) y[row0 + i] = b ? b[row0 + i] - plds[i] : plds[i];
+}
+
+// CG-loop SpMV on the panel layout: Ap = A p and block blk's p.Ap partial in partA[blk]
+template <typename VT>
+__global__ __launch_bounds__(kPanelBlock) void cg_spmv_panels(CgParams P) {
+    extern __shared__ double plds[];
+    __shared__ double red[kPanelBlock / 64];
+    CgState* st = P.st;
+    if (st->stop) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->k = st->k + 1;  // no other workgroup of this launch reads k
+    const int blk = blockIdx.x;
+    const double* __restrict__ p = P.p;
+    panel_rows<VT>(P.pan, P.n, blk, P.dsplit, p, plds);
+    const int row0 = blk * P.pan.block_rows, rows = min(P.pan.block_rows, P.n - row0);
+    double pap = 0.0;
+    for (int i = threadIdx.x; i < rows; i += kPanelBlock) {
+        const double y = plds[i];
+        P.Ap[row0 + i] = y;
+        pap += p[row0 + i] * y;
+    }
+    pap = block_sum_w<kPanelBlock / 64>(pap, red);
+    if (threadIdx.x == 0) P.partA[blk] = pap;
+}
+
 // col16[k] = col[k] - first row of k's tile; *overflow = 1 if any difference leaves int16
 __global__ __launch_bounds__(kBlock) void build_col16(int ntiles, const int* __restrict__ tile_row,
                                                       const int* __restrict__ rowptr, const int* __restrict__ col,
@@ -473,7 +584,15 @@ hipError_t launch_cg_setup(const CgParams& P, hipStream_t s) {
 }
 
 hipError_t launch_cg_iteration(const CgParams& P, bool f32, bool c16, hipStream_t s) {
-    XJ_DISPATCH_VC(f32, c16, (cg_spmv<VT, CT><<<P.ntiles, kBlock, 0, s>>>(P)));
+    if (P.pan_on) {
+        const size_t lds = (size_t)panel_lds_bytes(P.pan.block_rows);
+        if (f32)
+            cg_spmv_panels<float><<<P.pan.nb, kPanelBlock, lds, s>>>(P);
+        else
+            cg_spmv_panels<double><<<P.pan.nb, kPanelBlock, lds, s>>>(P);
+    } else {
+        XJ_DISPATCH_VC(f32, c16, (cg_spmv<VT, CT><<<P.ntiles, kBlock, 0, s>>>(P)));
+    }
     if (P.split_alpha) cg_alpha<<<P.nA2, kBlock, 0, s>>>(P);
     cg_update_r<<<P.G2, kBlock, 0, s>>>(P);
     cg_update_xp<<<P.G2, kBlock, 0, s>>>(P);
@@ -481,6 +600,14 @@ hipError_t launch_cg_iteration(const CgParams& P, bool f32, bool c16, hipStream_
 }
 
 hipError_t launch_spmv(const CsrDev& A, const double* x, double* y, const double* b, hipStream_t s) {
+    if (A.pan_on) {
+        const size_t lds = (size_t)panel_lds_bytes(A.pan.block_rows);
+        if (A.val_f32)
+            spmv_panels<float><<<A.pan.nb, kPanelBlock, lds, s>>>(A.pan, A.n, A.dsplit, x, y, b);
+        else
+            spmv_panels<double><<<A.pan.nb, kPanelBlock, lds, s>>>(A.pan, A.n, A.dsplit, x, y, b);
+        return hipGetLastError();
+    }
     if (A.ntiles <= 0) return hipSuccess;
     const void* col = A.col16 ? (const void*)A.col16 : (const void*)A.col;
     XJ_DISPATCH_VC(A.val_f32, A.col16 != nullptr,
@@ -525,6 +652,17 @@ hipError_t launch_gather_f64(int64_t nnz, const int64_t* perm, const double* src
 
 hipError_t launch_gather_f32(int64_t nnz, const int64_t* perm, const double* src, float* dst, hipStream_t s) {
     gather_values<float, double><<<grid_for(nnz), kBlock, 0, s>>>(nnz, perm, src, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_vals(int64_t nnz, const int64_t* perm, const void* src, void* dst, bool f32, hipStream_t s) {
+    if (nnz <= 0) return hipSuccess;
+    if (f32)
+        gather_values<float, float><<<grid_for(nnz), kBlock, 0, s>>>(nnz, perm, static_cast<const float*>(src),
+                                                                     static_cast<float*>(dst));
+    else
+        gather_values<double, double><<<grid_for(nnz), kBlock, 0, s>>>(nnz, perm, static_cast<const double*>(src),
+                                                                       static_cast<double*>(dst));
     return hipGetLastError();
 }
 

@@ -43,19 +43,21 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Sum over the workgroup; every thread returns the identical value.
-// red must hold kWaves doubles of LDS.  Contains two __syncthreads.
-__device__ __forceinline__ double block_sum(double v, double* red) {
+// Sum over a workgroup of NW wavefronts; every thread returns the identical
+// value.  red must hold NW doubles of LDS.  Contains two __syncthreads.
+template <int NW>
+__device__ __forceinline__ double block_sum_w(double v, double* red) {
     v = wave_sum(v);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0) red[w] = v;
     __syncthreads();
     double s = red[0];
 #pragma unroll
-    for (int i = 1; i < kWaves; ++i) s += red[i];
+    for (int i = 1; i < NW; ++i) s += red[i];
     __syncthreads();
     return s;
 }
+__device__ __forceinline__ double block_sum(double v, double* red) { return block_sum_w<kWaves>(v, red); }
 
 // Sum of part[0..G) in a fixed order, identical in every workgroup.
 __device__ __forceinline__ double grid_total(const double* part, int G, double* red) {

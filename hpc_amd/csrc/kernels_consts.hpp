@@ -11,3 +11,17 @@ constexpr int kTileRowsHost = 256;   // rows per tile (one lane per row)
 // more entries is a tile of its own, summed by the whole workgroup.
 constexpr int kTilePlanNnz = kTileNnzHost - 4;
 }  // namespace xj
+
+namespace xj {
+// Panel-swept layout (spmv_panels / cg_spmv_panels in jpcg.hip): column
+// panels of kPanelColsHost columns, so an entry's column is stored as a
+// uint16 offset in its panel; row blocks of at most kPanelRowsMaxHost rows,
+// whose running sums stay in LDS (uint16 row offsets); workgroups of
+// kPanelBlockHost threads stage kPanelChunkHost entries at a time.
+constexpr int kPanelColsHost = 65536;
+constexpr int kPanelRowsMaxHost = 4096;
+constexpr int kPanelBlockHost = 1024;
+constexpr int kPanelChunkHost = 2048;
+// LDS bytes of one panel workgroup: row sums + staged products + staged row offsets
+constexpr int panel_lds_bytes(int block_rows) { return block_rows * 8 + kPanelChunkHost * 10; }
+}  // namespace xj

@@ -8,6 +8,21 @@ namespace xj {
 
 struct Record;
 
+// The panel-swept copy of an operator (see panel_rows in jpcg.hip): row
+// block b (block_rows rows) stores its entries of column panel p (65536
+// columns) as segment [seg[b*P+p], seg[b*P+p+1]), row-major with ascending
+// columns; val in the operator's value type, off = column - 65536*p,
+// lrow = row - block_rows*b.
+struct PanelDev {
+    int nb = 0;          // row blocks
+    int block_rows = 0;  // rows per block (the last block may be shorter)
+    int P = 0;           // column panels
+    const int* seg = nullptr;
+    const void* val = nullptr;
+    const uint16_t* off = nullptr;
+    const uint16_t* lrow = nullptr;
+};
+
 // A CSR operator resident in HBM plus its tile plan.
 //   rowptr[n+1], col[nnz]  int32
 //   col16[nnz]             optional compact column stream: col - first row of the entry's tile
@@ -26,6 +41,8 @@ struct CsrDev {
     const double* dsplit = nullptr;
     const int* tile_row = nullptr;
     int ntiles = 0;
+    PanelDev pan;           // panel-swept copy, used by the SpMV kernels when pan_on
+    bool pan_on = false;
 };
 
 struct JpcgParams {
@@ -60,11 +77,14 @@ struct CgState {
     int pad;
 };
 
-// Arguments of the multi-kernel loop: SpMV (one workgroup per tile, p.Ap
-// partial per tile), optionally a one-workgroup alpha reduction, then two
-// vector kernels (G2 workgroups, grid-stride).
+// Arguments of the multi-kernel loop: SpMV (one workgroup per tile, or per
+// row block of the panel layout, with one p.Ap partial each), optionally a
+// reduction of those partials, then two vector kernels (G2 workgroups,
+// grid-stride).  ntiles counts the SpMV workgroups, hence the partials.
 struct CgParams {
     int n = 0, ntiles = 0, nnz = 0, G2 = 0;
+    int pan_on = 0;                      // 1: the SpMV runs on the panel layout `pan`
+    PanelDev pan;
     int split_alpha = 0;                 // 1: cg_alpha first folds the ntiles partials into nA2
     int nA2 = 0;
     double* partA2 = nullptr;            // nA2 doubles
@@ -100,6 +120,8 @@ int dot_blocks(int n);
 hipError_t launch_dot(int n, const double* a, const double* b, double* part, double* out, hipStream_t s);
 hipError_t launch_gather_f64(int64_t nnz, const int64_t* perm, const double* src, double* dst, hipStream_t s);
 hipError_t launch_gather_f32(int64_t nnz, const int64_t* perm, const double* src, float* dst, hipStream_t s);
+// dst[k] = src[perm[k]] in the operator's value type (fp32 when f32)
+hipError_t launch_gather_vals(int64_t nnz, const int64_t* perm, const void* src, void* dst, bool f32, hipStream_t s);
 hipError_t launch_axpy(int n, double a, const double* d, double* x, hipStream_t s);
 hipError_t launch_sddmm(const CsrDev& A, const double* lam, const double* x, double* g, hipStream_t s);
 

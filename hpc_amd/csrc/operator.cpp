@@ -33,6 +33,60 @@ std::vector<int32_t> plan_tiles(int32_t n, const int32_t* rp) {
     return t;
 }
 
+int panel_block_rows(int32_t n) {
+    // largest power of two up to kPanelRowsMaxHost that still gives >= 512 blocks
+    int br = kPanelRowsMaxHost;
+    while (br > 256 && (int64_t)n < (int64_t)br * 512) br >>= 1;
+    return br;
+}
+
+int64_t panel_segments(int32_t n, int block_rows) {
+    const int64_t nb = ((int64_t)n + block_rows - 1) / block_rows;
+    const int64_t P = ((int64_t)n + kPanelColsHost - 1) / kPanelColsHost;
+    return nb * P;
+}
+
+PanelPlan plan_panels(int32_t n, const int32_t* rp, const int32_t* col, int block_rows) {
+    PanelPlan pl;
+    pl.block_rows = block_rows;
+    pl.nb = (int)(((int64_t)n + block_rows - 1) / block_rows);
+    pl.P = (int)(((int64_t)n + kPanelColsHost - 1) / kPanelColsHost);
+    const int64_t nseg = (int64_t)pl.nb * pl.P;
+    const int64_t nnz = rp[n];
+    pl.seg.assign((size_t)nseg + 1, 0);
+    for (int32_t i = 0; i < n; ++i) {
+        const int64_t base = (int64_t)(i / block_rows) * pl.P;
+        int last = 0;
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+            const int32_t c = col[k];
+            if (c < 0 || c >= n) throw invalid("column index out of range.");
+            const int p = c / kPanelColsHost;
+            if (p < last) pl.ordered = false;
+            last = p;
+            ++pl.seg[(size_t)(base + p + 1)];
+        }
+    }
+    for (int64_t q = 0; q < nseg; ++q) pl.seg[(size_t)q + 1] += pl.seg[(size_t)q];
+    std::vector<int32_t> cur(pl.seg.begin(), pl.seg.end() - 1);
+    pl.off.resize((size_t)nnz);
+    pl.lrow.resize((size_t)nnz);
+    pl.perm.resize((size_t)nnz);
+    for (int32_t i = 0; i < n; ++i) {
+        const int b = i / block_rows;
+        const int64_t base = (int64_t)b * pl.P;
+        const uint16_t lr = (uint16_t)(i - b * block_rows);
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+            const int32_t c = col[k];
+            const int p = c / kPanelColsHost;
+            const int32_t q = cur[(size_t)(base + p)]++;
+            pl.off[(size_t)q] = (uint16_t)(c - p * kPanelColsHost);
+            pl.lrow[(size_t)q] = lr;
+            pl.perm[(size_t)q] = k;
+        }
+    }
+    return pl;
+}
+
 Operator::Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t* col, const void* val, bool val_f32,
                    const double* dsplit, const int32_t* rowptr_host, hipStream_t s) {
     if (n <= 0) throw invalid("Wrong dimension size.");
@@ -102,9 +156,123 @@ Operator::Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t*
             A_.col16 = col16_.ptr;
         }
     }
+    // Columns too spread for the int16 stream: when x is far larger than an
+    // XCD's L2, try the panel layout and keep it if it is clearly faster.
+    if (nnz > 0 && !has_col16_ && n >= kPanelAutoRows) {
+        if (build_panels(s)) {
+            const float t_tiles = time_spmv(false, s);
+            const float t_pan = time_spmv(true, s);
+            pan_auto_ = t_pan > 0.0f && t_tiles > 0.0f && t_pan < 0.85f * t_tiles;
+            A_.pan_on = pan_auto_;
+            if (!pan_auto_) free_panels();
+        }
+    }
     hip_check(persistent_grid(val_f32, A_.col16 != nullptr, &G_), "persistent kernel occupancy");
     part_.ensure((size_t)5 * G_ + 4096);
     hip_check(hipStreamSynchronize(s), "operator set-up");
+}
+
+bool Operator::build_panels(hipStream_t s) {
+    if (has_pan_) return true;
+    const int32_t n = A_.n;
+    const int br = panel_block_rows(n);
+    if (panel_segments(n, br) > kPanelMaxSegments) return false;
+    std::vector<int32_t> rp((size_t)n + 1), col((size_t)A_.nnz);
+    hip_check(hipMemcpyAsync(rp.data(), A_.rowptr, sizeof(int32_t) * rp.size(), hipMemcpyDeviceToHost, s),
+              "copy row pointers");
+    hip_check(hipMemcpyAsync(col.data(), A_.col, sizeof(int32_t) * col.size(), hipMemcpyDeviceToHost, s),
+              "copy columns");
+    hip_check(hipStreamSynchronize(s), "copy pattern");
+    PanelPlan pl = plan_panels(n, rp.data(), col.data(), br);
+    if (!pl.ordered) return false;
+    try {
+        pseg_.ensure(pl.seg.size());
+        poff_.ensure(pl.off.size());
+        plrow_.ensure(pl.lrow.size());
+        pperm_.ensure(pl.perm.size());
+        pval_.ensure(A_.val_f32 ? ((size_t)A_.nnz + 1) / 2 : (size_t)A_.nnz);
+        partA_.ensure((size_t)std::max(A_.ntiles, pl.nb));
+    } catch (const Error& e) {
+        if (e.status != XJPCG_STATUS_ALLOC_FAILED) throw;
+        free_panels();  // the panel layout is optional: keep the tiles
+        return false;
+    }
+    hip_check(hipMemcpyAsync(pseg_.ptr, pl.seg.data(), sizeof(int32_t) * pl.seg.size(), hipMemcpyHostToDevice, s),
+              "upload panel segments");
+    hip_check(hipMemcpyAsync(poff_.ptr, pl.off.data(), sizeof(uint16_t) * pl.off.size(), hipMemcpyHostToDevice, s),
+              "upload panel columns");
+    hip_check(hipMemcpyAsync(plrow_.ptr, pl.lrow.data(), sizeof(uint16_t) * pl.lrow.size(), hipMemcpyHostToDevice, s),
+              "upload panel rows");
+    hip_check(hipMemcpyAsync(pperm_.ptr, pl.perm.data(), sizeof(int64_t) * pl.perm.size(), hipMemcpyHostToDevice, s),
+              "upload panel map");
+    A_.pan.nb = pl.nb;
+    A_.pan.block_rows = pl.block_rows;
+    A_.pan.P = pl.P;
+    A_.pan.seg = pseg_.ptr;
+    A_.pan.val = pval_.ptr;
+    A_.pan.off = poff_.ptr;
+    A_.pan.lrow = plrow_.ptr;
+    has_pan_ = true;
+    pan_stale_ = true;
+    hip_check(hipStreamSynchronize(s), "upload panel layout");  // host plan buffers end with this scope
+    return true;
+}
+
+void Operator::free_panels() {
+    pseg_.release();
+    poff_.release();
+    plrow_.release();
+    pperm_.release();
+    pval_.release();
+    A_.pan = PanelDev();
+    A_.pan_on = false;
+    has_pan_ = false;
+}
+
+void Operator::refresh_panels(hipStream_t s) const {
+    if (!A_.pan_on || !pan_stale_) return;
+    hip_check(launch_gather_vals(A_.nnz, pperm_.ptr, A_.val, pval_.ptr, A_.val_f32, s), "panel values");
+    pan_stale_ = false;
+}
+
+float Operator::time_spmv(bool panels, hipStream_t s) {
+    A_.pan_on = panels;
+    refresh_panels(s);
+    hip_check(hipMemsetAsync(r_.ptr, 0, sizeof(double) * (size_t)A_.n, s), "clear timing input");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hip_check(hipEventCreate(&e0), "event");
+    hipError_t e = hipEventCreate(&e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) {
+        for (int i = 0; i < 2 && e == hipSuccess; ++i) e = launch_spmv(A_, r_.ptr, Ap_.ptr, nullptr, s);
+        if (e == hipSuccess) e = hipEventRecord(e0, s);
+        for (int i = 0; i < 5 && e == hipSuccess; ++i) e = launch_spmv(A_, r_.ptr, Ap_.ptr, nullptr, s);
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    }
+    (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    hip_check(e, "time SpMV layouts");
+    return ms;
+}
+
+void Operator::set_layout(int layout, hipStream_t s) {
+    bool want;
+    if (layout == kLayoutAuto)
+        want = pan_auto_;
+    else if (layout == kTiles)
+        want = false;
+    else if (layout == kPanels)
+        want = true;
+    else
+        throw invalid("layout must be 0 (auto), 1 (tiles) or 2 (panels).");
+    if (want && !has_pan_ && !build_panels(s))
+        throw invalid("the panel layout needs each row's columns in ascending order and fewer than 2^26 "
+                      "(row block, column panel) segments.");
+    if (want == A_.pan_on) return;
+    A_.pan_on = want;
+    drop_graph();
 }
 
 Operator::~Operator() {
@@ -127,6 +295,7 @@ void Operator::set_compact(bool on) {
 }
 
 void Operator::spmv(const double* x, double* y, hipStream_t s) const {
+    refresh_panels(s);
     hip_check(launch_spmv(A_, x, y, nullptr, s), "spmv launch");
 }
 
@@ -138,6 +307,7 @@ double Operator::dot(const double* a, const double* b, int n, hipStream_t s) {
 }
 
 double Operator::residual(const double* b, const double* x, double* r, hipStream_t s) {
+    refresh_panels(s);
     hip_check(launch_spmv(A_, x, r, b, s), "residual launch");
     return dot(r, r, A_.n, s);
 }
@@ -150,9 +320,10 @@ SolveResult Operator::solve(const double* diag, const double* b, double* x, uint
                             void* records, hipStream_t s) {
     if (diag == nullptr || b == nullptr || x == nullptr) throw invalid("Vector is nullptr.");
     if (max_iter == 0xFFFFFFFFu) throw invalid("maxIter too large.");
+    refresh_panels(s);
     int p = path_;
     if (p == kAuto) p = kMultiKernel;
-    if (!coop_ok_) p = kMultiKernel;
+    if (!coop_ok_ || A_.pan_on) p = kMultiKernel;  // the persistent kernel runs the tile SpMV only
     last_path_ = p;
     if (p == kPersistent) return solve_persistent(diag, b, x, max_iter, tol, records, s);
     return solve_multi(diag, b, x, max_iter, tol, records, s);
@@ -173,10 +344,12 @@ CgParams Operator::cg_params(const double* diag, const double* b, double* x, uin
                              void* records) const {
     CgParams P;
     P.n = A_.n;
-    P.ntiles = A_.ntiles;
+    P.pan_on = A_.pan_on ? 1 : 0;
+    P.pan = A_.pan;
+    P.ntiles = A_.pan_on ? A_.pan.nb : A_.ntiles;  // SpMV workgroups = p.Ap partials
     P.nnz = (int)A_.nnz;
     P.G2 = G2_;
-    P.split_alpha = A_.ntiles > kMaxRedundantPartials ? 1 : 0;
+    P.split_alpha = P.ntiles > kMaxRedundantPartials ? 1 : 0;
     P.nA2 = kAlphaBlocks;
     P.partA2 = partA2_.ptr;
     P.rowptr = A_.rowptr;
@@ -239,7 +412,7 @@ SolveResult Operator::solve_multi(const double* diag, const double* b, double* x
     // The iteration kernels read b, diag and tol only through the set-up, so a
     // graph captured for (x, records, max_rounds, values) replays for any b.
     if (graph_ok_ && (!gexec_ || gkey_.x != P.x || gkey_.records != P.records || gkey_.max_rounds != P.max_rounds ||
-                      gkey_.val != P.val || gkey_.col != P.col))
+                      gkey_.val != P.val || gkey_.col != P.col || gkey_.pan_on != P.pan_on))
         build_graph(P);
     // Enqueue iterations in chunks; after each chunk the stop flag is copied to
     // pinned memory.  The host checks chunk c-1's flag while chunk c is queued,

@@ -58,6 +58,22 @@ struct DevBuf {
 // Greedy row tiling (see tile_spmv in kernels.hpp): returns tile_row[ntiles+1].
 std::vector<int32_t> plan_tiles(int32_t n, const int32_t* rowptr_host);
 
+// Host plan of the panel-swept layout (PanelDev in launch.hpp): a stable
+// counting sort of the CSR entries by (row block, column panel).  perm[q] is
+// the CSR index of layout entry q.  Throws INVALID_VALUE for a column outside
+// [0, n); `ordered` is false when some row's columns are not in ascending
+// panel order (the layout would then change that row's summation order).
+struct PanelPlan {
+    int block_rows = 0, nb = 0, P = 0;
+    bool ordered = true;
+    std::vector<int32_t> seg;
+    std::vector<uint16_t> off, lrow;
+    std::vector<int64_t> perm;
+};
+int panel_block_rows(int32_t n);
+int64_t panel_segments(int32_t n, int block_rows);
+PanelPlan plan_panels(int32_t n, const int32_t* rowptr_host, const int32_t* col_host, int block_rows);
+
 struct SolveResult {
     int64_t executed = 0;  // iterations run (K)
     uint32_t iter = 0;     // reported count, min(K, maxIter)
@@ -83,13 +99,23 @@ class Operator {
     int grid() const { return G_; }
     bool compact() const { return A_.col16 != nullptr; }
     // point the operator at another value array of the same type and length
-    // (the pattern and tile plan stay); used when only A's values change
+    // (the pattern and tile plan stay); used when only A's values change, and
+    // with the same pointer after values changed in place, so that the panel
+    // layout's own copy of the values is refreshed before its next use
     void set_values(const void* val) {
         if (val == nullptr && A_.nnz > 0) throw invalid("Matrix is nullptr.");
         if (reinterpret_cast<uintptr_t>(val) & 15u) throw invalid("value array must be 16-byte aligned.");
         A_.val = val;
+        pan_stale_ = true;
         drop_graph();
     }
+    // SpMV layout: kTiles (row tiles over the CSR arrays) or kPanels (the
+    // panel-swept copy, for columns scattered over x).  kLayoutAuto restores
+    // the choice made at creation: panels when they were measured at least
+    // 15% faster on an operator whose columns do not fit the int16 stream.
+    enum Layout { kLayoutAuto = 0, kTiles = 1, kPanels = 2 };
+    void set_layout(int layout, hipStream_t s);
+    int layout() const { return A_.pan_on ? kPanels : kTiles; }
     // use (or stop using) the int16 column stream when it exists
     void set_compact(bool on);
 
@@ -126,9 +152,18 @@ class Operator {
                        void* records) const;
     void build_graph(const CgParams& P);
     void drop_graph();
+    // panel layout: build (host plan + upload), release, refresh values, time
+    bool build_panels(hipStream_t s);
+    void free_panels();
+    void refresh_panels(hipStream_t s) const;
+    float time_spmv(bool panels, hipStream_t s);
     const void* col_stream() const { return A_.col16 ? (const void*)A_.col16 : (const void*)A_.col; }
 
     static constexpr int kChunk = 8;  // iterations per captured graph
+    // panel layout considered from this many rows (x = 2 MiB, beyond what the
+    // tile kernel's gathers keep in L2 once columns scatter)
+    static constexpr int32_t kPanelAutoRows = 1 << 18;
+    static constexpr int64_t kPanelMaxSegments = 1ll << 26;
 
     CsrDev A_;
     DevBuf<int32_t> tiles_;
@@ -144,6 +179,13 @@ class Operator {
     DevBuf<unsigned long long> bar_;
     DevBuf<int> abort_;
     DevBuf<double> aux_[2];
+    DevBuf<int32_t> pseg_;
+    DevBuf<uint16_t> poff_, plrow_;
+    DevBuf<int64_t> pperm_;
+    DevBuf<double> pval_;            // nnz values of the operator's type, in panel order
+    bool has_pan_ = false;
+    bool pan_auto_ = false;          // creation-time choice
+    mutable bool pan_stale_ = false; // panel values differ from A_.val
     double* host_result_ = nullptr;  // pinned
     int* host_flags_ = nullptr;      // pinned
     hipStream_t work_ = nullptr;     // graphs are captured and replayed here

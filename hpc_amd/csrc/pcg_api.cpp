@@ -333,6 +333,7 @@ class PcgHandle {
                   "upload values");
         hip_check(launch_gather_f64(pat_nnz_, d_perm_.ptr, d_raw_.ptr, d_val_.ptr, stream_), "place values");
         hip_check(hipStreamSynchronize(stream_), "place values");
+        if (op_) op_->set_values(d_val_.ptr);  // values changed in place: refresh the operator's own copies
     }
 
     mutable XJPCG_Status_t last_status_ = XJPCG_STATUS_SUCCESS;

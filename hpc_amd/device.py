@@ -110,6 +110,20 @@ class CsrOperator:
         check_dev(lib().xjpcg_op_set_compact(self._h, -1, ctypes.byref(act)))
         return bool(act.value)
 
+    LAYOUT_AUTO, LAYOUT_TILES, LAYOUT_PANELS = 0, 1, 2
+
+    def set_layout(self, layout: int) -> int:
+        """SpMV layout: 0 the creation-time choice, 1 row tiles, 2 panel-swept copy; returns the layout in use."""
+        act = ctypes.c_int()
+        check_dev(lib().xjpcg_op_set_layout(self._h, int(layout), _stream(), ctypes.byref(act)))
+        return act.value
+
+    @property
+    def layout(self) -> int:
+        act = ctypes.c_int()
+        check_dev(lib().xjpcg_op_set_layout(self._h, -1, None, ctypes.byref(act)))
+        return act.value
+
     PATH_AUTO, PATH_PERSISTENT, PATH_MULTI = 0, 1, 2
 
     def set_path(self, path: int):
@@ -201,3 +215,26 @@ def split_fp32(rowptr_h, col_h, val_h):
     v32 = val_h[keep].astype(np.float32)
     exact = bool(np.all(v32.astype(np.float64) == val_h[keep]))
     return rp, np.ascontiguousarray(col_h[keep], dtype=np.int32), v32, diag, exact
+
+
+def panel_plan(rowptr_h, col_h):
+    """Host-only panel layout of a CSR pattern (numpy arrays), as an operator would build it.
+
+    Returns dict(block_rows, nb, P, seg[nb*P+1], off[nnz] uint16, lrow[nnz] uint16, perm[nnz] int64, ordered).
+    """
+    import numpy as np
+    rowptr_h = np.ascontiguousarray(rowptr_h, dtype=np.int32)
+    col_h = np.ascontiguousarray(col_h, dtype=np.int32)
+    n = rowptr_h.size - 1
+    br, nb, P = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    check_dev(lib().xjpcg_panel_dims(n, ctypes.byref(br), ctypes.byref(nb), ctypes.byref(P)))
+    nnz = int(rowptr_h[-1])
+    seg = np.zeros(nb.value * P.value + 1, np.int32)
+    off = np.zeros(nnz, np.uint16)
+    lrow = np.zeros(nnz, np.uint16)
+    perm = np.zeros(nnz, np.int64)
+    ordered = ctypes.c_int()
+    check_dev(lib().xjpcg_panel_plan(n, rowptr_h.ctypes.data, col_h.ctypes.data, seg.ctypes.data, off.ctypes.data,
+                                     lrow.ctypes.data, perm.ctypes.data, ctypes.byref(ordered)))
+    return {"block_rows": br.value, "nb": nb.value, "P": P.value, "seg": seg, "off": off, "lrow": lrow,
+            "perm": perm, "ordered": bool(ordered.value)}

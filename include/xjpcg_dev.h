@@ -51,8 +51,32 @@ XJPCG_Status_t xjpcg_op_destroy(xjpcg_op_t* op);
 XJPCG_Status_t xjpcg_op_set_compact(xjpcg_op_t* op, int on, int* h_active);
 /* Rebind the operator to another device value array of the same type and
  * length; the pattern and row tiles are kept (the device-side counterpart of
- * the reference's KEEP_NZ_LAYOUT value refresh, signature.hpp:427-469). */
+ * the reference's KEEP_NZ_LAYOUT value refresh, signature.hpp:427-469).
+ * Call it also, with the same pointer, after changing values in place: an
+ * operator on the panel layout keeps its own copy of the values, refreshed
+ * on the stream of its next use. */
 XJPCG_Status_t xjpcg_op_set_values(xjpcg_op_t* op, const void* d_val);
+/* SpMV layout (no counterpart in the reference, whose SpMV streams its own
+ * partitioned format from HBM): 1 = row tiles over the CSR arrays;
+ * 2 = panel-swept copy for matrices whose columns scatter over x (row blocks
+ * of up to 4096 rows keep their sums in LDS and sweep 65536-column panels of
+ * x in order, so the gathered window stays in L2; 12 bytes per entry plus an
+ * 8-byte value map; rows are still summed left to right).  At creation an
+ * operator with at least 2^18 rows whose columns do not fit the int16 stream
+ * times both and keeps panels when they are at least 15% faster;
+ * layout = 0 restores that choice, layout < 0 only queries.  Layout 2 needs
+ * each row's columns in ascending order.  *h_active (may be NULL) receives
+ * the layout in use.  Building the panel copy synchronises `stream`. */
+XJPCG_Status_t xjpcg_op_set_layout(xjpcg_op_t* op, int layout, void* stream, int* h_active);
+/* Host-only view of the panel layout an operator would build (no device
+ * calls): xjpcg_panel_dims gives the rows per block, the row blocks and the
+ * column panels for n rows; xjpcg_panel_plan fills seg[nb*P+1], off[nnz],
+ * lrow[nnz] and perm[nnz] (layout entry q holds CSR entry perm[q]) from a host
+ * CSR.  *h_ordered = 0 when some row's columns are not in ascending panel
+ * order.  No counterpart in the reference; for inspection and tests. */
+XJPCG_Status_t xjpcg_panel_dims(int32_t n, int32_t* block_rows, int32_t* nb, int32_t* npanels);
+XJPCG_Status_t xjpcg_panel_plan(int32_t n, const int32_t* h_rowptr, const int32_t* h_col, int32_t* seg,
+                                uint16_t* off, uint16_t* lrow, int64_t* perm, int* h_ordered);
 /* n, nnz, number of row tiles, grid of the persistent kernel */
 XJPCG_Status_t xjpcg_op_info(const xjpcg_op_t* op, int32_t* n, int64_t* nnz, int32_t* ntiles, int32_t* grid);
 

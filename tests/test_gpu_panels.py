@@ -1,0 +1,122 @@
+"""Panel-swept SpMV layout on the GPU: SpMV bit-exact against the oracle (every row, long ones included),
+solve parity, value refresh, split-diagonal fp32 operators."""
+import sys
+
+import numpy as np
+import pytest
+
+from hpc_amd import workloads
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(m):
+    print(m, flush=True)
+    sys.stdout.flush()
+
+
+def _t(cuda, a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
+
+
+def scattered_csr(n, seed, max_deg=40):
+    rng = np.random.default_rng(seed)
+    deg = rng.integers(0, max_deg, n)
+    deg[rng.random(n) < 0.001] = 3000
+    rows = np.repeat(np.arange(n), deg)
+    cols = rng.integers(0, n, rows.size)
+    vals = rng.standard_normal(rows.size)
+    o = np.lexsort((cols, rows))
+    rp, col, val, _ = oracle.coo_to_csr(n, rows[o], cols[o], vals[o])
+    return rp, col, val
+
+
+def test_panel_spmv_bitexact_all_rows(cuda):
+    from hpc_amd import device
+    n = 300_000
+    rp, col, val = scattered_csr(n, 12)
+    op = device.CsrOperator(_t(cuda, rp), _t(cuda, col), _t(cuda, val), rowptr_host=rp)
+    _say(f"n={n} nnz={col.size} creation-time layout {op.layout}")
+    assert op.set_layout(2) == 2
+    xv = workloads.uniform(5, 3, n, -1, 1)
+    x = _t(cuda, xv)
+    ref = oracle.spmv(rp, col, val, xv, threads=8)
+    y = op.spmv(x).cpu().numpy()
+    assert np.array_equal(y, ref)               # left-to-right in every row, long rows too
+    # residual on the same layout
+    bv = workloads.uniform(5, 4, n, -1, 1)
+    r, rr = op.residual(_t(cuda, bv), x)
+    assert np.array_equal(r.cpu().numpy(), bv - ref)
+    # tiles agree on every row short enough for a tile
+    assert op.set_layout(1) == 1
+    yt = op.spmv(x).cpu().numpy()
+    short = np.diff(rp) <= 2044
+    assert np.array_equal(yt[short], ref[short])
+
+
+def test_panel_solve_matches_oracle(cuda):
+    from hpc_amd import device
+    A = workloads.graph_laplacian(300_000, seed=7)
+    b = workloads.uniform(7, 1, A.n, -1, 1)
+    op = device.CsrOperator(_t(cuda, A.rowptr), _t(cuda, A.col), _t(cuda, A.val), rowptr_host=A.rowptr)
+    diag, bd = _t(cuda, A.diag), _t(cuda, b)
+    tol, maxit = 1e-10, 20000
+    ref = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, b, maxit, tol, threads=8)
+    out = {}
+    for layout in (1, 2):
+        assert op.set_layout(layout) == layout
+        x, info = op.solve(diag, bd, maxit, tol)
+        xh = x.cpu().numpy()
+        e = np.linalg.norm(xh - ref["x"]) / np.linalg.norm(ref["x"])
+        _say(f"layout {layout}: {info.executed} iterations (oracle {ref['executed']}), rel err {e:.2e}")
+        assert abs(info.executed - ref["executed"]) <= 1
+        assert e <= 1e-10 and info.residual < tol
+        out[layout] = xh
+    x2, _ = op.solve(diag, bd, maxit, tol)
+    assert np.array_equal(x2.cpu().numpy(), out[2])   # deterministic on the panel layout
+
+
+def test_panel_value_refresh(cuda):
+    """set_values with a new array and after an in-place change reaches the panel copy."""
+    from hpc_amd import device
+    n = 280_000
+    rp, col, val = scattered_csr(n, 13, max_deg=12)
+    v = _t(cuda, val)
+    op = device.CsrOperator(_t(cuda, rp), _t(cuda, col), v, rowptr_host=rp)
+    assert op.set_layout(2) == 2
+    xv = workloads.uniform(6, 1, n, -1, 1)
+    x = _t(cuda, xv)
+    y1 = op.spmv(x).cpu().numpy()
+    v2 = v * 2.0
+    op.set_values(v2)
+    assert np.array_equal(op.spmv(x).cpu().numpy(), 2.0 * y1)
+    v2.mul_(0.25)                               # in place, then announce it
+    op.set_values(v2)
+    assert np.array_equal(op.spmv(x).cpu().numpy(), 0.5 * y1)
+
+
+def test_panel_split_fp32_and_mixed(cuda):
+    """fp32 off-diagonal values + fp64 split diagonal on the panel layout: same SpMV as the tiles (rows are
+    short), and the mixed refined solve reaches the tolerance."""
+    import torch
+    from hpc_amd import device
+    A = workloads.graph_laplacian(300_000, seed=8)
+    b = workloads.uniform(8, 1, A.n, -1, 1)
+    srp, scol, sv32, sdiag, exact = device.split_fp32(A.rowptr, A.col, A.val)
+    assert exact
+    inner = device.CsrOperator(_t(cuda, srp), _t(cuda, scol), _t(cuda, sv32), dsplit=_t(cuda, sdiag), rowptr_host=srp)
+    outer = device.CsrOperator(_t(cuda, srp), _t(cuda, scol), _t(cuda, sv32), dsplit=_t(cuda, sdiag), rowptr_host=srp)
+    x = _t(cuda, workloads.uniform(8, 2, A.n, -1, 1))
+    assert inner.set_layout(1) == 1
+    yt = inner.spmv(x).clone()
+    assert inner.set_layout(2) == 2
+    yp = inner.spmv(x)
+    short = torch.from_numpy(np.diff(srp) <= 2044).to(cuda)
+    assert bool((yt[short] == yp[short]).all())
+    assert outer.set_layout(2) == 2
+    xm, info = device.solve_mixed(outer, inner, _t(cuda, A.diag), _t(cuda, b), 1e-10)
+    true = np.linalg.norm(b - A.to_scipy() @ xm.cpu().numpy()) / np.linalg.norm(b)
+    _say(f"mixed on panels: outer {info.outer} inner {info.inner_total} true res {true:.2e}")
+    assert true < 1e-10
