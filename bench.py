@@ -269,6 +269,23 @@ def bench_batched(args):
     print(json.dumps(out), flush=True)
 
 
+def batched_profile(args):
+    """Short run for hardware-counter passes on the batched kernels: the batch of --batch-count systems
+    x --nrhs right-hand sides, 10 fp64 iterations (set-up plus 10 graph-free iteration launches)."""
+    import torch
+    from hpc_amd import batched, workloads
+    dev = torch.device("cuda:0")
+    mats, rhs = workloads.c4_systems(args.batch_count, 5000, 400_000, args.nrhs, seed=3)
+    asm = batched.assemble(mats)
+    b = torch.from_numpy(np.concatenate(rhs)).to(dev)
+    diag = torch.from_numpy(asm["diag"]).to(dev)
+    B = batched.Batch.from_host(asm, args.nrhs, dev)
+    x, info = B.solve(diag, b, 9, 1e-30)
+    torch.cuda.synchronize()
+    print(json.dumps({"N": B.N, "nnz": int(asm["col"].size), "R": args.nrhs, "launched": info.launched,
+                      "config": B.info()}), flush=True)
+
+
 def bench_autograd(args):
     """Differentiable solve on 2D variable-coefficient diffusion 512^2: forward JPCG, then backward
     (adjoint JPCG + SDDMM) for L = 0.5 ||x - x_target||^2, fp64."""
@@ -321,7 +338,8 @@ def main():
     ap.add_argument("--spmv-repeats", type=int, default=100)
     ap.add_argument("--inner-tol", type=float, default=0.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", choices=["full", "spmv-profile", "graph", "batched", "autograd"], default="full",
+    ap.add_argument("--mode", choices=["full", "spmv-profile", "graph", "batched", "autograd", "batched-profile"],
+                    default="full",
                     help="spmv-profile: one 256 MiB device copy (counter calibration) then 10 SpMV launches, "
                          "for rocprofv3 --pmc runs; graph / batched / autograd: the graph-Laplacian, batched "
                          "and differentiable workloads, one JSON line each")
@@ -336,11 +354,12 @@ def main():
     if args.mode == "spmv-profile":
         spmv_profile(args)
         return
-    if args.mode in ("graph", "batched", "autograd"):
+    if args.mode in ("graph", "batched", "autograd", "batched-profile"):
         import torch
         assert torch.cuda.is_available(), "bench.py needs a GPU"
         torch.cuda.set_device(0)
-        {"graph": bench_graph, "batched": bench_batched, "autograd": bench_autograd}[args.mode](args)
+        {"graph": bench_graph, "batched": bench_batched, "autograd": bench_autograd,
+         "batched-profile": batched_profile}[args.mode](args)
         return
 
     import torch

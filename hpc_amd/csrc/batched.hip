@@ -37,12 +37,23 @@ __device__ __forceinline__ double group_sum(double v, double* red) {
     return s;
 }
 
-// Sum of part[t*R + j] over the tiles t of system s, fixed order.
+// Sum of part[t*R + j] over the tiles t of system s, fixed order.  Each lane
+// issues four loads before adding them in order, so a large system's partials
+// cost a quarter of the dependent round trips.
 template <int R>
 __device__ __forceinline__ double system_sum(const double* part, int t0, int t1, double* red) {
     const int j = threadIdx.x % R, g = threadIdx.x / R, G = blockDim.x / R;
     double s = 0.0;
-    for (int t = t0 + g; t < t1; t += G) s += part[(size_t)t * R + j];
+    int t = t0 + g;
+    for (; t + 3 * G < t1; t += 4 * G) {
+        const double a0 = part[(size_t)t * R + j], a1 = part[(size_t)(t + G) * R + j];
+        const double a2 = part[(size_t)(t + 2 * G) * R + j], a3 = part[(size_t)(t + 3 * G) * R + j];
+        s += a0;
+        s += a1;
+        s += a2;
+        s += a3;
+    }
+    for (; t < t1; t += G) s += part[(size_t)t * R + j];
     return group_sum<R>(s, red);
 }
 
@@ -315,16 +326,16 @@ static void iteration_r(const BatchParams& P, bool f32, int block, hipStream_t s
         bspmm_dot<float, R><<<P.T, block, 0, s>>>(P);
     else
         bspmm_dot<double, R><<<P.T, block, 0, s>>>(P);
-    bsys_alpha<R><<<P.S, 256, 0, s>>>(P);
+    bsys_alpha<R><<<P.S, kMaxBlock, 0, s>>>(P);
     bupdate_r<R><<<P.T, block, 0, s>>>(P);
-    bsys_beta<R><<<P.S, 256, 0, s>>>(P);
+    bsys_beta<R><<<P.S, kMaxBlock, 0, s>>>(P);
     bupdate_xp<R><<<P.T, block, 0, s>>>(P);
 }
 
 template <int R>
 static void setup_r(const BatchParams& P, int block, hipStream_t s) {
     bsetup<R><<<P.T, block, 0, s>>>(P);
-    bsys_setup<R><<<P.S, 256, 0, s>>>(P);
+    bsys_setup<R><<<P.S, kMaxBlock, 0, s>>>(P);
 }
 
 template <int R>
@@ -339,7 +350,7 @@ static void spmm_r(const BatchParams& P, bool f32, int block, const double* x, d
 template <int R>
 static void norms_r(const BatchParams& P, int block, const double* v, double* part, double* out, hipStream_t s) {
     btile_norm<R><<<P.T, block, 0, s>>>(P, v, part);
-    bsys_norm<R><<<P.S, 256, 0, s>>>(P, part, out);
+    bsys_norm<R><<<P.S, kMaxBlock, 0, s>>>(P, part, out);
 }
 
 template <int R>
