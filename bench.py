@@ -8,7 +8,7 @@ generated on the host, uploaded, and resident in HBM before any timed region.
   value       fp64 SpMV algorithmic HBM GB/s (B = 12 nnz + 4(n+1) + 16 n per launch),
               100 launches timed with HIP events on the launch stream
   steps       timed JPCG solves per precision (x0 = 0 to ||r||/||b|| < 1e-10)
-  ms_per_step fp64 JPCG time-to-1e-10 (mean of the timed solves)
+  ms_per_step fp64 JPCG time-to-1e-10 (median of the timed solves)
   jpcg        fp64 and mixed-refined times, iterations, true residuals
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 160] [--no-cpu]
@@ -61,14 +61,55 @@ def cpu_baseline(A, budget_s=15.0):
     return out, threads_all
 
 
-def measured_traffic(grid, compact):
-    """HBM bytes per SpMV launch from the committed counter summary (only for the workload it measured)."""
+LAYOUT_NAMES = {1: "tiles", 2: "panels", 3: "patterns"}
+LAYOUT_KERNELS = {1: "spmv_tiles<double, %s columns>", 2: "spmv_panels<double>", 3: "spmv_pattern<double>"}
+TRAFFIC_PROFILES = {"tiles": "spmv_tiles_stencil27_160.md", "patterns": "spmv_pattern_stencil27_160.md"}
+
+
+def measured_traffic(grid, layout_name, compact=True):
+    """HBM bytes per SpMV launch from the committed counter summary of that kernel (only for the workload it
+    measured: the 27-point 160^3 operator; the tile kernel on its int16 column stream)."""
     import re
-    path = os.path.join(ROOT, "profiles", "spmv_tiles_stencil27_160.md")
-    if grid != 160 or not compact or not os.path.exists(path):
-        return None
+    fname = TRAFFIC_PROFILES.get(layout_name)
+    if grid != 160 or fname is None or (layout_name == "tiles" and not compact):
+        return None, None
+    path = os.path.join(ROOT, "profiles", fname)
+    if not os.path.exists(path):
+        return None, None
     m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", open(path).read(), re.M)
-    return int(m.group(1)) if m else None
+    return (int(m.group(1)), "profiles/" + fname) if m else (None, None)
+
+
+def achievable_copy(dev, stream, nbytes=2 << 30, reps=10):
+    """Achievable HBM rate: the library's stream kernel copying nbytes, (read + write bytes) / time, median of
+    reps."""
+    import torch
+    from hpc_amd import device
+    src = torch.empty(nbytes // 8, dtype=torch.float64, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    device.stream_copy(src, dst)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        device.stream_copy(src, dst)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    assert bool((dst == 1.0).all())
+    del src, dst
+    return 2 * nbytes / (float(np.median(ts)) * 1e-3) / 1e9
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def spmv_profile(args):
@@ -391,36 +432,67 @@ def main():
 
     stream = torch.cuda.current_stream()
 
-    # ---- SpMV: per-launch HIP events on the launch stream
+    # ---- SpMV: per-launch HIP events on the launch stream, every layout that applies; the reported one is
+    # the layout the operator chose for itself at creation (the one its solves use)
     xs = torch.ones(A.n, dtype=torch.float64, device=dev)
     ys = torch.empty_like(xs)
-    for _ in range(max(3, args.warmup)):
-        op.spmv(xs, ys)
-    torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(args.spmv_repeats)]
-    for e0, e1 in evs:
-        e0.record(stream)
-        op.spmv(xs, ys)
-        e1.record(stream)
-    torch.cuda.synchronize()
-    spmv_ms = np.array([e0.elapsed_time(e1) for e0, e1 in evs])
     nb = spmv_bytes(A.n, A.nnz)
+    chosen = op.layout
+
+    def spmv_series(repeats):
+        for _ in range(max(3, args.warmup)):
+            op.spmv(xs, ys)
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(repeats)]
+        for e0, e1 in evs:
+            e0.record(stream)
+            op.spmv(xs, ys)
+            e1.record(stream)
+        torch.cuda.synchronize()
+        return np.array([e0.elapsed_time(e1) for e0, e1 in evs])
+
+    layouts = {}
+    for lay in (1, 3):
+        try:
+            op.set_layout(lay)
+        except Exception:
+            continue
+        ms = spmv_series(args.spmv_repeats)
+        layouts[LAYOUT_NAMES[lay]] = {"median_us": round(float(np.median(ms)) * 1e3, 2),
+                                      "GBs": round(nb / (float(np.median(ms)) * 1e-3) / 1e9, 1)}
+    op.set_layout(0)
+    spmv_ms = spmv_series(args.spmv_repeats)
     spmv_med = float(np.median(spmv_ms))
     spmv_gbs = nb / (spmv_med * 1e-3) / 1e9
-    log(f"SpMV fp64: median {spmv_med * 1e3:.1f} us  ->  {spmv_gbs:.0f} GB/s")
-    # fp32-value SpMV of the inner operator
-    for _ in range(3):
-        inner.spmv(xs, ys)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(args.spmv_repeats):
-        inner.spmv(xs, ys)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    spmv32_ms = e0.elapsed_time(e1) / args.spmv_repeats
+    log(f"SpMV fp64 ({LAYOUT_NAMES[chosen]}): median {spmv_med * 1e3:.1f} us  ->  {spmv_gbs:.0f} GB/s; "
+        f"per layout {layouts}")
+    # fp32-value SpMV of the inner operator, every layout that applies, then its own choice
+    def spmv32_time():
+        for _ in range(3):
+            inner.spmv(xs, ys)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.spmv_repeats):
+            inner.spmv(xs, ys)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.spmv_repeats
+
+    inner_layouts = {}
+    for lay in (1, 3):
+        try:
+            inner.set_layout(lay)
+        except Exception:
+            continue
+        inner_layouts[LAYOUT_NAMES[lay]] = round(spmv32_time() * 1e3, 2)
+    inner.set_layout(0)
+    spmv32_ms = spmv32_time()
+    log(f"fp32-value SpMV per layout (us): {inner_layouts}, chosen {LAYOUT_NAMES[inner.layout]}")
     nb32 = 8 * inner.nnz + 4 * (A.n + 1) + 24 * A.n
     log(f"SpMV fp32-values+fp64 diag: {spmv32_ms * 1e3:.1f} us  ->  {nb32 / (spmv32_ms * 1e-3) / 1e9:.0f} GB/s")
+
+    copy_gbs = achievable_copy(dev, stream)
+    log(f"achievable copy rate: {copy_gbs:.0f} GB/s")
 
     # ---- JPCG fp64
     x = torch.empty_like(b)
@@ -440,7 +512,7 @@ def main():
     _, rr = op.residual(b, x)
     bb = float(torch.dot(b, b))
     true_res64 = (rr / bb) ** 0.5
-    fp64_ms = float(np.mean(times))
+    fp64_ms = float(np.median(times))
     it_bytes = iter_bytes(A.n, A.nnz) * info.executed
     log(f"JPCG fp64: {fp64_ms:.1f} ms, {info.executed} iterations, recursive res {info.residual:.2e}, "
         f"true res {true_res64:.2e}, {it_bytes / (fp64_ms * 1e-3) / 1e9:.0f} GB/s (B_iter formula)")
@@ -458,7 +530,7 @@ def main():
         e1.record(stream)
         torch.cuda.synchronize()
         mtimes.append(e0.elapsed_time(e1))
-    mixed_ms = float(np.mean(mtimes))
+    mixed_ms = float(np.median(mtimes))
     _, rrm = op.residual(b, xm)
     true_res_m = (rrm / bb) ** 0.5
     err_vs64 = float(torch.linalg.norm(xm - x) / torch.linalg.norm(x))
@@ -471,9 +543,13 @@ def main():
         res, th = cpu_baseline(A)
         cpu = {"value": round(res[th][0], 2), "unit": "GB/s", "cores": th, "kind": "port",
                "sample": f"oracle CSR SpMV on the same 27-pt 160^3 matrix, {res[th][1]} repeats on {th} threads "
-                         f"(1 thread: {res[1][0]:.2f} GB/s, {res[1][1]} repeats); same byte formula"}
+                         f"(1 thread: {res[1][0]:.2f} GB/s, {res[1][1]} repeats); same byte formula; host "
+                         f"{cpu_model()}, {os.cpu_count()} logical CPUs visible"}
         log(f"CPU: {cpu}")
 
+    traffic, traffic_src = measured_traffic(args.grid, LAYOUT_NAMES[chosen], op.compact)
+    kernel = LAYOUT_KERNELS[chosen] % ("int16-delta" if op.compact else "int32") if chosen == 1 \
+        else LAYOUT_KERNELS[chosen]
     out = {
         "metric": "JPCG time-to-1e-10 residual & SpMV HBM GB/s: mixed-refined vs fp64, 1 GPU",
         "value": round(spmv_gbs, 1),
@@ -492,23 +568,28 @@ def main():
                    "parallelism": "single GPU"},
         "spmv": {"repeats": args.spmv_repeats, "median_us": round(spmv_med * 1e3, 2),
                  "mean_us": round(float(spmv_ms.mean()) * 1e3, 2), "bytes_per_launch": nb,
-                 "fp32_values_us": round(spmv32_ms * 1e3, 2),
+                 "layout": LAYOUT_NAMES[chosen], "per_layout": layouts,
+                 "fp32_values_us": round(spmv32_ms * 1e3, 2), "fp32_values_per_layout_us": inner_layouts,
                  "fp32_values_GBs": round(nb32 / (spmv32_ms * 1e-3) / 1e9, 1)},
         "jpcg": {
             "fp64": {"time_to_tol_ms": round(fp64_ms, 3), "wall_ms": round(t_fp64_wall * 1e3, 3),
+                     "layout": LAYOUT_NAMES[chosen],
                      "iterations": info.executed, "recursive_res": info.residual, "true_res": true_res64,
                      "iter_GBs_formula": round(it_bytes / (fp64_ms * 1e-3) / 1e9, 1)},
             "mixed": {"time_to_tol_ms": round(mixed_ms, 3), "outer": minfo.outer, "inner_total": minfo.inner_total,
+                      "inner_layout": LAYOUT_NAMES[inner.layout],
                       "inner_tol": args.inner_tol, "true_res": true_res_m, "rel_diff_vs_fp64": err_vs64},
             "speedup_mixed_vs_fp64": round(fp64_ms / mixed_ms, 3),
         },
         "roofline": {"bound": "hbm", "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(spmv_gbs / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(args.grid, op.compact),
-                     "kernel": "spmv_tiles<double, %s columns>" % ("int16-delta" if op.compact else "int32"),
+                     "traffic": traffic,
+                     "kernel": kernel,
                      "algorithmic_bytes_per_launch": nb,
-                     "traffic_source": "profiles/spmv_tiles_stencil27_160.md (rocprofv3 FETCH_SIZE/WRITE_SIZE, "
-                                       "copy-calibrated)"},
+                     "achievable_copy_GBs": round(copy_gbs, 1),
+                     "physical_GBs": round(traffic / (spmv_med * 1e-3) / 1e9, 1) if traffic else None,
+                     "traffic_source": (traffic_src + " (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)")
+                     if traffic_src else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -44,6 +44,9 @@ _SIGNATURES = {
     "xjpcg_panel_dims": (c_int, [c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
     "xjpcg_panel_plan": (c_int, [c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  POINTER(c_int)]),
+    "xjpcg_tile_plan": (c_int, [c_int32, c_void_p, c_void_p, POINTER(c_int32)]),
+    "xjpcg_stream_copy": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
+    "xjpcg_pattern_plan": (c_int, [c_int32, c_void_p, c_void_p, c_void_p, c_void_p, POINTER(c_int)]),
     "xjpcg_spmv": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "xjpcg_residual": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, POINTER(c_double), c_void_p]),
     "xjpcg_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_double, POINTER(c_uint32),

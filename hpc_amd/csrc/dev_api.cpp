@@ -99,6 +99,33 @@ XJPCG_Status_t xjpcg_panel_plan(int32_t n, const int32_t* h_rowptr, const int32_
     });
 }
 
+XJPCG_Status_t xjpcg_tile_plan(int32_t n, const int32_t* h_rowptr, int32_t* tiles, int32_t* h_ntiles) {
+    return guarded([&] {
+        if (n <= 0) throw xj::invalid("Wrong dimension size.");
+        if (!h_rowptr || !tiles) throw xj::invalid("Matrix is nullptr.");
+        const std::vector<int32_t> t = xj::plan_tiles(n, h_rowptr);
+        std::copy(t.begin(), t.end(), tiles);
+        if (h_ntiles) *h_ntiles = (int32_t)t.size() - 1;
+    });
+}
+
+XJPCG_Status_t xjpcg_pattern_plan(int32_t n, const int32_t* h_rowptr, const int32_t* h_col, uint32_t* mask,
+                                  int32_t* offs, int* h_ok) {
+    return guarded([&] {
+        if (n <= 0) throw xj::invalid("Wrong dimension size.");
+        if (!h_rowptr || !mask || !offs || (h_rowptr[n] > 0 && !h_col)) throw xj::invalid("Matrix is nullptr.");
+        const std::vector<int32_t> t = xj::plan_tiles(n, h_rowptr);
+        std::vector<uint32_t> m;
+        std::vector<int32_t> o;
+        const bool ok = xj::plan_patterns(n, h_rowptr, h_col, t, m, o);
+        if (ok) {
+            std::copy(m.begin(), m.end(), mask);
+            std::copy(o.begin(), o.end(), offs);
+        }
+        if (h_ok) *h_ok = ok ? 1 : 0;
+    });
+}
+
 XJPCG_Status_t xjpcg_op_set_layout(xjpcg_op_t* op, int layout, void* stream, int* h_active) {
     return guarded([&] {
         xj::Operator& o = impl_of(op);
@@ -124,6 +151,16 @@ XJPCG_Status_t xjpcg_op_set_path(xjpcg_op_t* op, int path, int* h_last) {
         if (path < 0) return;  // query only
         if (path > 2) throw xj::invalid("path must be 0, 1 or 2.");
         o.set_path(path);
+    });
+}
+
+XJPCG_Status_t xjpcg_stream_copy(const double* d_src, double* d_dst, int64_t n, void* stream) {
+    return guarded([&] {
+        if (n < 0) throw xj::invalid("length must not be negative.");
+        if (n > 0 && (!d_src || !d_dst)) throw xj::invalid("Vector is nullptr.");
+        if ((reinterpret_cast<uintptr_t>(d_src) | reinterpret_cast<uintptr_t>(d_dst)) & 15u)
+            throw xj::invalid("arrays must be 16-byte aligned.");
+        xj::hip_check(xj::launch_stream_copy(n, d_src, d_dst, as_stream(stream)), "stream copy launch");
     });
 }
 

@@ -10,6 +10,7 @@
 //  dot_partials / sum_partials   fixed-order two-pass dot product
 //  gather_values     KEEP_NZ_LAYOUT value refresh through the stored permutation
 //  sddmm_tiles       g_k = -lam[row_k] * x[col_k] on the sparsity pattern
+//  spmv_pattern / cg_spmv_pattern SpMV on column-offset pattern tiles (stencils)
 //  spmv_panels / cg_spmv_panels   SpMV on the panel-swept layout (operators
 //                    whose columns scatter over x), standalone and in the CG loop
 //
@@ -89,7 +90,8 @@ __global__ __launch_bounds__(kBlock) void jpcg_persistent(JpcgParams P) {
             for (int t = sw.first; t < sw.end; t += sw.step) {
                 double y;
                 int row;
-                tile_spmv<VT, CT>(P.tile_row[t], P.tile_row[t + 1], P.nnz, P.rowptr, col, val, P.dsplit, p, lds,
+                tile_spmv<VT, CT>(P.tile_row[t], P.tile_row[t + 1], P.tile_k[t], P.tile_k[t + 1], P.nnz, P.rowptr,
+                                  col, val, P.dsplit, p, lds,
                                   red, &y, &row);
                 if (row >= 0) {
                     Ap[row] = y;
@@ -231,7 +233,8 @@ __global__ __launch_bounds__(kBlock) void cg_spmv(CgParams P) {
     const double* __restrict__ p = P.p;
     double y;
     int row;
-    tile_spmv<VT, CT>(P.tile_row[t], P.tile_row[t + 1], P.nnz, P.rowptr, static_cast<const CT*>(P.col),
+    tile_spmv<VT, CT>(P.tile_row[t], P.tile_row[t + 1], P.tile_k[t], P.tile_k[t + 1], P.nnz, P.rowptr,
+                      static_cast<const CT*>(P.col),
                       static_cast<const VT*>(P.val), P.dsplit, p, lds, red, &y, &row);
     double pap = 0.0;
     if (row >= 0) {
@@ -240,6 +243,42 @@ __global__ __launch_bounds__(kBlock) void cg_spmv(CgParams P) {
     }
     pap = block_sum(pap, red);
     if (threadIdx.x == 0) P.partA[t] = pap;
+}
+
+// CG-loop SpMV on column-offset pattern tiles: logical workgroup w runs tiles
+// 2w and 2w+1 (consecutive rows); the p.Ap partial of both is partA[w]
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void cg_spmv_pattern(CgParams P) {
+    __shared__ double lds[kTileNnz];
+    __shared__ double red[kWaves];
+    __shared__ int loffs[32];
+    CgState* st = P.st;
+    if (st->stop) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->k = st->k + 1;  // no other workgroup of this launch reads k
+    const int w = xcd_remap(blockIdx.x, gridDim.x);
+    const double* __restrict__ p = P.p;
+    const VT* __restrict__ val = static_cast<const VT*>(P.val);
+    PatternTile<VT> T[kPatternTiles];
+#pragma unroll
+    for (int u = 0; u < kPatternTiles; ++u) {
+        const int t = w * kPatternTiles + u;
+        if (t < P.tiles)
+            pattern_tile_load<VT>(T[u], t, P.tile_row, P.tile_k, P.nnz, P.rowptr, P.pmask, P.poffs, val);
+    }
+    double pap = 0.0;
+#pragma unroll
+    for (int u = 0; u < kPatternTiles; ++u) {
+        if (T[u].t < 0) break;
+        double y;
+        int row;
+        pattern_tile_compute<VT>(T[u], P.dsplit, p, lds, loffs, &y, &row);
+        if (row >= 0) {
+            P.Ap[row] = y;
+            pap += p[row] * y;
+        }
+    }
+    pap = block_sum(pap, red);
+    if (threadIdx.x == 0) P.partA[w] = pap;
 }
 
 // second reduction level for many tiles: workgroup b sums a contiguous slice
@@ -329,6 +368,7 @@ __global__ __launch_bounds__(kBlock) void cg_update_xp(CgParams P) {
 
 template <typename VT, typename CT>
 __global__ __launch_bounds__(kBlock) void spmv_tiles(int ntiles, int nnz, const int* __restrict__ tile_row,
+                                                     const int* __restrict__ tile_k,
                                                      const int* __restrict__ rowptr, const CT* __restrict__ col,
                                                      const VT* __restrict__ val, const double* __restrict__ dsplit,
                                                      const double* __restrict__ x, double* __restrict__ y,
@@ -339,7 +379,8 @@ __global__ __launch_bounds__(kBlock) void spmv_tiles(int ntiles, int nnz, const 
     if (t >= ntiles) return;
     double v;
     int row;
-    tile_spmv<VT, CT>(tile_row[t], tile_row[t + 1], nnz, rowptr, col, val, dsplit, x, lds, red, &v, &row);
+    tile_spmv<VT, CT>(tile_row[t], tile_row[t + 1], tile_k[t], tile_k[t + 1], nnz, rowptr, col, val, dsplit, x, lds,
+                      red, &v, &row);
     if (row >= 0) y[row] = b ? b[row] - v : v;
 }
 
@@ -370,6 +411,7 @@ static_assert(kPanelChunk % kPanelBlock == 0, "whole chunk entries per lane");
 template <typename VT>
 __device__ __forceinline__ void panel_rows(const PanelDev& A, int n, int b, const double* __restrict__ dsplit,
                                            const double* __restrict__ x, double* acc) {
+#pragma clang fp contract(off)
     constexpr int J = kPanelChunk / kPanelBlock;
     double* prod = acc + A.block_rows;
     uint16_t* lr = reinterpret_cast<uint16_t*>(prod + kPanelChunk);
@@ -452,6 +494,32 @@ __global__ __launch_bounds__(kPanelBlock) void cg_spmv_panels(CgParams P) {
     if (threadIdx.x == 0) P.partA[blk] = pap;
 }
 
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void spmv_pattern(int ntiles, int nnz, const int* __restrict__ tile_row,
+                                                       const int* __restrict__ tile_k,
+                                                       const int* __restrict__ rowptr, const uint32_t* __restrict__ pmask,
+                                                       const int* __restrict__ poffs, const VT* __restrict__ val,
+                                                       const double* __restrict__ dsplit, const double* __restrict__ x,
+                                                       double* __restrict__ y, const double* __restrict__ b) {
+    __shared__ double lds[kTileNnz];
+    __shared__ int loffs[32];
+    const int w = xcd_remap(blockIdx.x, gridDim.x);
+    PatternTile<VT> T[kPatternTiles];
+#pragma unroll
+    for (int u = 0; u < kPatternTiles; ++u) {
+        const int t = w * kPatternTiles + u;
+        if (t < ntiles) pattern_tile_load<VT>(T[u], t, tile_row, tile_k, nnz, rowptr, pmask, poffs, val);
+    }
+#pragma unroll
+    for (int u = 0; u < kPatternTiles; ++u) {
+        if (T[u].t < 0) break;
+        double v;
+        int row;
+        pattern_tile_compute<VT>(T[u], dsplit, x, lds, loffs, &v, &row);
+        if (row >= 0) y[row] = b ? b[row] - v : v;
+    }
+}
+
 // col16[k] = col[k] - first row of k's tile; *overflow = 1 if any difference leaves int16
 __global__ __launch_bounds__(kBlock) void build_col16(int ntiles, const int* __restrict__ tile_row,
                                                       const int* __restrict__ rowptr, const int* __restrict__ col,
@@ -491,6 +559,19 @@ __global__ __launch_bounds__(kBlock) void gather_values(int64_t nnz, const int64
                                                         const TI* __restrict__ src, TO* __restrict__ dst) {
     for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * kBlock)
         dst[k] = (TO)src[perm[k]];
+}
+
+// dst = src for n doubles: 16-byte non-temporal loads and stores, grid-stride
+// (the achievable-bandwidth reference: every byte read once, written once)
+__global__ __launch_bounds__(kBlock) void stream_copy(int64_t n, const double* __restrict__ src,
+                                                      double* __restrict__ dst) {
+    typedef unsigned int W __attribute__((ext_vector_type(4)));
+    const int64_t n2 = n >> 1;
+    const W* __restrict__ s = reinterpret_cast<const W*>(src);
+    W* __restrict__ d = reinterpret_cast<W*>(dst);
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n2; i += (int64_t)gridDim.x * kBlock)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(s + i), d + i);
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) dst[n - 1] = src[n - 1];
 }
 
 // x = x + a*d  (iterative-refinement update)
@@ -590,6 +671,11 @@ hipError_t launch_cg_iteration(const CgParams& P, bool f32, bool c16, hipStream_
             cg_spmv_panels<float><<<P.pan.nb, kPanelBlock, lds, s>>>(P);
         else
             cg_spmv_panels<double><<<P.pan.nb, kPanelBlock, lds, s>>>(P);
+    } else if (P.pat_on) {
+        if (f32)
+            cg_spmv_pattern<float><<<pattern_groups(P.tiles), kBlock, 0, s>>>(P);
+        else
+            cg_spmv_pattern<double><<<pattern_groups(P.tiles), kBlock, 0, s>>>(P);
     } else {
         XJ_DISPATCH_VC(f32, c16, (cg_spmv<VT, CT><<<P.ntiles, kBlock, 0, s>>>(P)));
     }
@@ -609,9 +695,22 @@ hipError_t launch_spmv(const CsrDev& A, const double* x, double* y, const double
         return hipGetLastError();
     }
     if (A.ntiles <= 0) return hipSuccess;
+    if (A.pat_on) {
+        if (A.val_f32)
+            spmv_pattern<float><<<pattern_groups(A.ntiles), kBlock, 0, s>>>(A.ntiles, (int)A.nnz, A.tile_row, A.tile_k,
+                                                                            A.rowptr, A.pmask,
+                                                            A.poffs, static_cast<const float*>(A.val), A.dsplit, x, y, b);
+        else
+            spmv_pattern<double><<<pattern_groups(A.ntiles), kBlock, 0, s>>>(A.ntiles, (int)A.nnz, A.tile_row, A.tile_k,
+                                                                             A.rowptr, A.pmask,
+                                                             A.poffs, static_cast<const double*>(A.val), A.dsplit, x, y,
+                                                             b);
+        return hipGetLastError();
+    }
     const void* col = A.col16 ? (const void*)A.col16 : (const void*)A.col;
     XJ_DISPATCH_VC(A.val_f32, A.col16 != nullptr,
-                   (spmv_tiles<VT, CT><<<A.ntiles, kBlock, 0, s>>>(A.ntiles, (int)A.nnz, A.tile_row, A.rowptr,
+                   (spmv_tiles<VT, CT><<<A.ntiles, kBlock, 0, s>>>(A.ntiles, (int)A.nnz, A.tile_row, A.tile_k,
+                                                                   A.rowptr,
                                                                    static_cast<const CT*>(col),
                                                                    static_cast<const VT*>(A.val), A.dsplit, x, y,
                                                                    b)));
@@ -663,6 +762,15 @@ hipError_t launch_gather_vals(int64_t nnz, const int64_t* perm, const void* src,
     else
         gather_values<double, double><<<grid_for(nnz), kBlock, 0, s>>>(nnz, perm, static_cast<const double*>(src),
                                                                        static_cast<double*>(dst));
+    return hipGetLastError();
+}
+
+hipError_t launch_stream_copy(int64_t n, const double* src, double* dst, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int64_t g = (n / 2 + kBlock - 1) / kBlock;
+    if (g > 8192) g = 8192;
+    if (g < 1) g = 1;
+    stream_copy<<<(unsigned)g, kBlock, 0, s>>>(n, src, dst);
     return hipGetLastError();
 }
 

@@ -153,13 +153,15 @@ __device__ __forceinline__ void load_run(const T* __restrict__ p, T (&out)[E]) {
     __builtin_memcpy(out, &w, sizeof(W));
 }
 
+// k0 = rowptr[r0], k1 = rowptr[r1] come from the per-tile table tile_k, so
+// they load together with r0 and r1 instead of after them.
 template <typename VT, typename CT>
-__device__ __forceinline__ void tile_spmv(int r0, int r1, int nnz, const int* __restrict__ rowptr,
+__device__ __forceinline__ void tile_spmv(int r0, int r1, int k0, int k1, int nnz, const int* __restrict__ rowptr,
                                           const CT* __restrict__ col, const VT* __restrict__ val,
                                           const double* __restrict__ dsplit, const double* __restrict__ x,
                                           double* lds, double* red, double* y, int* row) {
+#pragma clang fp contract(off)
     const int tid = threadIdx.x;
-    const int k0 = rowptr[r0], k1 = rowptr[r1];
     if (k1 - k0 > kTilePlanNnz) {
         double s = 0.0;
         for (int k = k0 + tid; k < k1; k += kBlock) s += (double)ld_stream(val + k) * x[col_at(col, k, r0)];
@@ -248,6 +250,140 @@ __device__ __forceinline__ void tile_spmv(int r0, int r1, int nnz, const int* __
     }
     __syncthreads();
 }
+
+// Column-offset pattern tiles.  When every tile of the plan uses at most 32
+// distinct column offsets (col - row), as stencil operators do, the operator
+// keeps, instead of a column index per entry, each tile's ascending offsets
+// (poffs[t*32 + b]) and one 32-bit mask per row (bit b set: the row has an
+// entry at column row + poffs[t*32 + b]).  With strictly ascending columns in
+// every row, a row's stored entries are its set bits in ascending order.
+// Lane t first expands row t's mask into the tile's column indices in LDS;
+// then the tile runs as in tile_spmv (16-byte value loads, all gathers of a
+// lane in flight together, products in LDS, lane-per-row sums left to
+// right), reading each entry's column from its LDS slot before the product
+// overwrites it.  Same products and sums as tile_spmv, with no column stream.
+// What one lane needs of a pattern tile before the tile's compute starts
+// (all of it loaded up front, so that a workgroup can have the loads of its
+// next tile in flight while it computes the current one).
+template <typename VT>
+struct PatternTile {
+    static constexpr int E = 16 / (int)sizeof(VT);
+    static constexpr int kGroups = kTileNnz / (kBlock * E);
+    int t = -1, r0 = 0, r1 = 0, k0 = 0, k1 = 0;
+    int a = 0, b = 0;      // this lane's row: entries [a, b) relative to k0
+    uint32_t m = 0;        // this lane's row mask
+    int off = 0;           // lanes < 32: the tile's offset #lane
+    VT v[kGroups][E];      // this lane's share of the value stream
+};
+
+template <typename VT>
+__device__ __forceinline__ void pattern_tile_load(PatternTile<VT>& T, int t, const int* __restrict__ tile_row,
+                                                  const int* __restrict__ tile_k, int nnz,
+                                                  const int* __restrict__ rowptr, const uint32_t* __restrict__ pmask,
+                                                  const int* __restrict__ poffs, const VT* __restrict__ val) {
+    constexpr int E = PatternTile<VT>::E, kGroups = PatternTile<VT>::kGroups;
+    const int tid = threadIdx.x;
+    T.t = t;
+    T.r0 = tile_row[t];
+    T.r1 = tile_row[t + 1];
+    T.k0 = tile_k[t];
+    T.k1 = tile_k[t + 1];
+    const int i = T.r0 + tid;
+    const bool mine = tid < T.r1 - T.r0;
+    T.a = mine ? rowptr[i] - T.k0 : 0;
+    T.b = mine ? rowptr[i + 1] - T.k0 : 0;
+    T.m = mine ? pmask[i] : 0u;
+    T.off = tid < 32 ? poffs[t * 32 + tid] : 0;
+    const int k0 = T.k0, k1 = T.k1;
+    const int k0a = k0 & ~(E - 1);
+    if (k1 > k0) {
+        if (((k1 + E - 1) & ~(E - 1)) <= nnz) {
+#pragma unroll
+            for (int g = 0; g < kGroups; ++g) {
+                int base = k0a + (g * kBlock + tid) * E;
+                if (base >= k1) base = k0a;
+                load_run<VT, E>(val + base, T.v[g]);
+            }
+        } else {
+#pragma unroll
+            for (int g = 0; g < kGroups; ++g)
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const int kk = k0a + (g * kBlock + tid) * E + e;
+                    T.v[g][e] = ld_stream(val + (kk >= k0 && kk < k1 ? kk : k0));
+                }
+        }
+    }
+}
+
+// Compute of a loaded pattern tile; on return lanes with *row >= 0 hold
+// y[*row] in *y.  Starts and ends with a __syncthreads, so consecutive tiles
+// of a workgroup can reuse lds and loffs.
+template <typename VT>
+__device__ __forceinline__ void pattern_tile_compute(const PatternTile<VT>& T, const double* __restrict__ dsplit,
+                                                     const double* __restrict__ x, double* lds, int* loffs, double* y,
+                                                     int* row) {
+#pragma clang fp contract(off)
+    constexpr int E = PatternTile<VT>::E, kGroups = PatternTile<VT>::kGroups;
+    const int tid = threadIdx.x;
+    const int k0 = T.k0, k1 = T.k1, r0 = T.r0;
+    const int i = r0 + tid;
+    const bool mine = tid < T.r1 - r0;
+    __syncthreads();  // the previous tile of this workgroup is done with lds / loffs
+    if (tid < 32) loffs[tid] = T.off;
+    __syncthreads();
+    // columns are parked as doubles (exact below 2^53) in the slots the
+    // products reuse, so the column read and the product write of an entry
+    // are accesses of one type the compiler keeps in order
+    uint32_t m = T.m;
+    for (int j = T.a; m; ++j) {
+        lds[j] = (double)(i + loffs[__builtin_ctz(m)]);
+        m &= m - 1;
+    }
+    __syncthreads();
+    if (k1 > k0) {
+        const int k0a = k0 & ~(E - 1);
+        double gx[kGroups][E];
+#pragma unroll
+        for (int g = 0; g < kGroups; ++g)
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int k = k0a + (g * kBlock + tid) * E + e;
+                gx[g][e] = x[(k >= k0 && k < k1) ? (int)lds[k - k0] : r0];
+            }
+#pragma unroll
+        for (int g = 0; g < kGroups; ++g) {
+            const int base = k0a + (g * kBlock + tid) * E;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int k = base + e;
+                if (k >= k0 && k < k1) lds[k - k0] = (double)T.v[g][e] * gx[g][e];
+            }
+        }
+    }
+    __syncthreads();
+    if (mine) {
+        double s = dsplit ? dsplit[i] * x[i] : 0.0;
+        int j = T.a;
+        const int b = T.b;
+        for (; j + 4 <= b; j += 4) {
+            const double q0 = lds[j], q1 = lds[j + 1], q2 = lds[j + 2], q3 = lds[j + 3];
+            s += q0;
+            s += q1;
+            s += q2;
+            s += q3;
+        }
+        for (; j < b; ++j) s += lds[j];
+        *y = s;
+        *row = i;
+    } else {
+        *row = -1;
+    }
+}
+
+// Pattern tiles per workgroup: the loads of both are issued before the first
+// tile's compute (measured: 2 beats 1 and 4 for fp64 and fp32 values).
+constexpr int kPatternTiles = kPatternTilesHost;
 
 // Tiles of logical workgroup L in a grid of G for a tile loop: the logical
 // workgroups of one XCD (a contiguous range under xcd_remap) own one

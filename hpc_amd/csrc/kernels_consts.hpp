@@ -24,4 +24,8 @@ constexpr int kPanelBlockHost = 1024;
 constexpr int kPanelChunkHost = 2048;
 // LDS bytes of one panel workgroup: row sums + staged products + staged row offsets
 constexpr int panel_lds_bytes(int block_rows) { return block_rows * 8 + kPanelChunkHost * 10; }
+// column-offset pattern tiles run two tiles per workgroup; the CG loop keeps
+// one p.Ap partial per workgroup
+constexpr int kPatternTilesHost = 2;
+constexpr int pattern_groups(int ntiles) { return (ntiles + kPatternTilesHost - 1) / kPatternTilesHost; }
 }  // namespace xj

@@ -40,9 +40,13 @@ struct CsrDev {
     bool val_f32 = false;
     const double* dsplit = nullptr;
     const int* tile_row = nullptr;
+    const int* tile_k = nullptr;      // rowptr[tile_row[t]] for t = 0..ntiles
     int ntiles = 0;
     PanelDev pan;           // panel-swept copy, used by the SpMV kernels when pan_on
     bool pan_on = false;
+    const uint32_t* pmask = nullptr;  // column-offset pattern tiles (tile_spmv_pattern): mask per row
+    const int* poffs = nullptr;       // 32 offsets per tile
+    bool pat_on = false;
 };
 
 struct JpcgParams {
@@ -52,6 +56,7 @@ struct JpcgParams {
     const void* val = nullptr;
     const double* dsplit = nullptr;
     const int* tile_row = nullptr;
+    const int* tile_k = nullptr;
     const double* b = nullptr;
     const double* diag = nullptr;  // Jacobi diagonal supplied by the caller
     double *x = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr, *J = nullptr;
@@ -83,8 +88,12 @@ struct CgState {
 // grid-stride).  ntiles counts the SpMV workgroups, hence the partials.
 struct CgParams {
     int n = 0, ntiles = 0, nnz = 0, G2 = 0;
+    int tiles = 0;                       // row tiles of the plan (ntiles counts the partials)
     int pan_on = 0;                      // 1: the SpMV runs on the panel layout `pan`
     PanelDev pan;
+    int pat_on = 0;                      // 1: the SpMV runs on column-offset pattern tiles
+    const uint32_t* pmask = nullptr;
+    const int* poffs = nullptr;
     int split_alpha = 0;                 // 1: cg_alpha first folds the ntiles partials into nA2
     int nA2 = 0;
     double* partA2 = nullptr;            // nA2 doubles
@@ -93,6 +102,7 @@ struct CgParams {
     const void* val = nullptr;
     const double* dsplit = nullptr;
     const int* tile_row = nullptr;
+    const int* tile_k = nullptr;
     const double* b = nullptr;
     const double* diag = nullptr;
     double *x = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr, *J = nullptr;
@@ -123,6 +133,7 @@ hipError_t launch_gather_f32(int64_t nnz, const int64_t* perm, const double* src
 // dst[k] = src[perm[k]] in the operator's value type (fp32 when f32)
 hipError_t launch_gather_vals(int64_t nnz, const int64_t* perm, const void* src, void* dst, bool f32, hipStream_t s);
 hipError_t launch_axpy(int n, double a, const double* d, double* x, hipStream_t s);
+hipError_t launch_stream_copy(int64_t n, const double* src, double* dst, hipStream_t s);
 hipError_t launch_sddmm(const CsrDev& A, const double* lam, const double* x, double* g, hipStream_t s);
 
 }  // namespace xj

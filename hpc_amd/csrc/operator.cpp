@@ -33,6 +33,63 @@ std::vector<int32_t> plan_tiles(int32_t n, const int32_t* rp) {
     return t;
 }
 
+bool plan_patterns(int32_t n, const int32_t* rp, const int32_t* col, const std::vector<int32_t>& tiles,
+                   std::vector<uint32_t>& mask, std::vector<int32_t>& offs) {
+    const int nt = (int)tiles.size() - 1;
+    mask.assign((size_t)n, 0u);
+    offs.assign((size_t)nt * 32, 0);
+    std::vector<int32_t> ds;
+    ds.reserve(32);
+    // offset of the u-th entry of the previous row and its bit: stencil rows
+    // repeat their neighbour's pattern, so most entries skip the search
+    int32_t last_d[32], last_b[32];
+    for (int t = 0; t < nt; ++t) {
+        const int32_t r0 = tiles[(size_t)t], r1 = tiles[(size_t)t + 1];
+        if (rp[r1] - rp[r0] > kTilePlanNnz) return false;
+        ds.clear();
+        int nlast = 0;
+        for (int32_t i = r0; i < r1; ++i) {
+            const int len = rp[i + 1] - rp[i];
+            if (len > 32) return false;
+            int32_t prev = -1;
+            for (int u = 0; u < len; ++u) {
+                const int32_t c = col[rp[i] + u];
+                if (c < 0 || c >= n || c <= prev) return false;
+                prev = c;
+                const int32_t d = c - i;
+                if (!(u < nlast && last_d[u] == d) && std::find(ds.begin(), ds.end(), d) == ds.end()) {
+                    if (ds.size() == 32) return false;
+                    ds.push_back(d);
+                }
+                last_d[u] = d;
+            }
+            nlast = len;
+        }
+        std::sort(ds.begin(), ds.end());
+        for (size_t b = 0; b < ds.size(); ++b) offs[(size_t)t * 32 + b] = ds[b];
+        nlast = 0;
+        for (int32_t i = r0; i < r1; ++i) {
+            const int len = rp[i + 1] - rp[i];
+            uint32_t m = 0;
+            for (int u = 0; u < len; ++u) {
+                const int32_t d = col[rp[i] + u] - i;
+                int b;
+                if (u < nlast && last_d[u] == d) {
+                    b = last_b[u];
+                } else {
+                    b = (int)(std::lower_bound(ds.begin(), ds.end(), d) - ds.begin());
+                }
+                last_d[u] = d;
+                last_b[u] = b;
+                m |= 1u << b;
+            }
+            nlast = len;
+            mask[(size_t)i] = m;
+        }
+    }
+    return true;
+}
+
 int panel_block_rows(int32_t n) {
     // largest power of two up to kPanelRowsMaxHost that still gives >= 512 blocks
     int br = kPanelRowsMaxHost;
@@ -108,9 +165,15 @@ Operator::Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t*
     for (int32_t i = 0; i < n; ++i)
         if (rp[i + 1] < rp[i]) throw invalid("row pointers must not decrease.");
     std::vector<int32_t> t = plan_tiles(n, rp);
+    tile_plan_ = t;
+    std::vector<int32_t> tk(t.size());
+    for (size_t q = 0; q < t.size(); ++q) tk[q] = rp[t[q]];
     tiles_.ensure(t.size());
+    tilek_.ensure(tk.size());
     hip_check(hipMemcpyAsync(tiles_.ptr, t.data(), sizeof(int32_t) * t.size(), hipMemcpyHostToDevice, s),
               "upload tile plan");
+    hip_check(hipMemcpyAsync(tilek_.ptr, tk.data(), sizeof(int32_t) * tk.size(), hipMemcpyHostToDevice, s),
+              "upload tile entry offsets");
     A_.n = n;
     A_.nnz = nnz;
     A_.rowptr = rowptr;
@@ -119,6 +182,7 @@ Operator::Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t*
     A_.val_f32 = val_f32;
     A_.dsplit = dsplit;
     A_.tile_row = tiles_.ptr;
+    A_.tile_k = tilek_.ptr;
     A_.ntiles = (int)t.size() - 1;
     G2_ = std::min((n + 1023) / 1024, 2048);
     if (G2_ < 1) G2_ = 1;
@@ -156,34 +220,101 @@ Operator::Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t*
             A_.col16 = col16_.ptr;
         }
     }
-    // Columns too spread for the int16 stream: when x is far larger than an
-    // XCD's L2, try the panel layout and keep it if it is clearly faster.
-    if (nnz > 0 && !has_col16_ && n >= kPanelAutoRows) {
-        if (build_panels(s)) {
-            const float t_tiles = time_spmv(false, s);
-            const float t_pan = time_spmv(true, s);
-            pan_auto_ = t_pan > 0.0f && t_tiles > 0.0f && t_pan < 0.85f * t_tiles;
-            A_.pan_on = pan_auto_;
-            if (!pan_auto_) free_panels();
+    // Layout choice, timed on this operator: column-offset pattern tiles when
+    // every tile has at most 32 distinct offsets (stencils), and, when the
+    // columns are too spread for the int16 stream and x is far larger than an
+    // XCD's L2, the panel layout.  The faster layout stays (panels only when
+    // at least 15% faster than the tiles).
+    if (nnz > 0) {
+        std::vector<int32_t> rp_h, col_h;
+        copy_pattern_to_host(s, rp_h, col_h);
+        const float t_tiles = time_layout(kTiles, s);
+        float best = t_tiles;
+        if (build_patterns(s, rp_h.data(), col_h.data())) {
+            const float t_pat = time_layout(kPatterns, s);
+            if (t_pat > 0.0f && t_pat < best) {
+                best = t_pat;
+                auto_layout_ = kPatterns;
+            }
         }
+        if (!has_col16_ && n >= kPanelAutoRows && build_panels(s, rp_h.data(), col_h.data())) {
+            const float t_pan = time_layout(kPanels, s);
+            if (t_pan > 0.0f && t_pan < 0.85f * t_tiles && t_pan < best) {
+                best = t_pan;
+                auto_layout_ = kPanels;
+            }
+        }
+        if (auto_layout_ != kPanels) free_panels();
+        if (auto_layout_ != kPatterns) free_patterns();
+        A_.pan_on = auto_layout_ == kPanels;
+        A_.pat_on = auto_layout_ == kPatterns;
     }
     hip_check(persistent_grid(val_f32, A_.col16 != nullptr, &G_), "persistent kernel occupancy");
     part_.ensure((size_t)5 * G_ + 4096);
     hip_check(hipStreamSynchronize(s), "operator set-up");
 }
 
-bool Operator::build_panels(hipStream_t s) {
+void Operator::copy_pattern_to_host(hipStream_t s, std::vector<int32_t>& rp, std::vector<int32_t>& col) const {
+    rp.resize((size_t)A_.n + 1);
+    col.resize((size_t)A_.nnz);
+    hip_check(hipMemcpyAsync(rp.data(), A_.rowptr, sizeof(int32_t) * rp.size(), hipMemcpyDeviceToHost, s),
+              "copy row pointers");
+    if (!col.empty())
+        hip_check(hipMemcpyAsync(col.data(), A_.col, sizeof(int32_t) * col.size(), hipMemcpyDeviceToHost, s),
+                  "copy columns");
+    hip_check(hipStreamSynchronize(s), "copy pattern");
+}
+
+bool Operator::build_patterns(hipStream_t s, const int32_t* rp_host, const int32_t* col_host) {
+    if (has_pat_) return true;
+    if (pat_tried_) return false;
+    pat_tried_ = true;
+    std::vector<int32_t> rp_l, col_l;
+    if (!rp_host || !col_host) copy_pattern_to_host(s, rp_l, col_l);
+    const int32_t* rp = rp_host && col_host ? rp_host : rp_l.data();
+    const int32_t* col = rp_host && col_host ? col_host : col_l.data();
+    std::vector<uint32_t> mask;
+    std::vector<int32_t> offs;
+    if (!plan_patterns(A_.n, rp, col, tile_plan_, mask, offs)) return false;
+    try {
+        pmask_.ensure(mask.size());
+        poffs_.ensure(offs.size());
+    } catch (const Error& e) {
+        if (e.status != XJPCG_STATUS_ALLOC_FAILED) throw;
+        free_patterns();
+        return false;
+    }
+    hip_check(hipMemcpyAsync(pmask_.ptr, mask.data(), sizeof(uint32_t) * mask.size(), hipMemcpyHostToDevice, s),
+              "upload row masks");
+    hip_check(hipMemcpyAsync(poffs_.ptr, offs.data(), sizeof(int32_t) * offs.size(), hipMemcpyHostToDevice, s),
+              "upload tile offsets");
+    hip_check(hipStreamSynchronize(s), "upload pattern tiles");
+    A_.pmask = pmask_.ptr;
+    A_.poffs = poffs_.ptr;
+    has_pat_ = true;
+    return true;
+}
+
+void Operator::free_patterns() {
+    pmask_.release();
+    poffs_.release();
+    A_.pmask = nullptr;
+    A_.poffs = nullptr;
+    A_.pat_on = false;
+    has_pat_ = false;
+    pat_tried_ = false;
+}
+
+bool Operator::build_panels(hipStream_t s, const int32_t* rp_host, const int32_t* col_host) {
     if (has_pan_) return true;
     const int32_t n = A_.n;
     const int br = panel_block_rows(n);
     if (panel_segments(n, br) > kPanelMaxSegments) return false;
-    std::vector<int32_t> rp((size_t)n + 1), col((size_t)A_.nnz);
-    hip_check(hipMemcpyAsync(rp.data(), A_.rowptr, sizeof(int32_t) * rp.size(), hipMemcpyDeviceToHost, s),
-              "copy row pointers");
-    hip_check(hipMemcpyAsync(col.data(), A_.col, sizeof(int32_t) * col.size(), hipMemcpyDeviceToHost, s),
-              "copy columns");
-    hip_check(hipStreamSynchronize(s), "copy pattern");
-    PanelPlan pl = plan_panels(n, rp.data(), col.data(), br);
+    std::vector<int32_t> rp_l, col_l;
+    if (!rp_host || !col_host) copy_pattern_to_host(s, rp_l, col_l);
+    const int32_t* rp = rp_host && col_host ? rp_host : rp_l.data();
+    const int32_t* col = rp_host && col_host ? col_host : col_l.data();
+    PanelPlan pl = plan_panels(n, rp, col, br);
     if (!pl.ordered) return false;
     try {
         pseg_.ensure(pl.seg.size());
@@ -235,8 +366,10 @@ void Operator::refresh_panels(hipStream_t s) const {
     pan_stale_ = false;
 }
 
-float Operator::time_spmv(bool panels, hipStream_t s) {
-    A_.pan_on = panels;
+// time 5 SpMV launches on `layout` (after 2 warm-up launches); leaves that layout switched on
+float Operator::time_layout(int layout, hipStream_t s) {
+    A_.pan_on = layout == kPanels;
+    A_.pat_on = layout == kPatterns;
     refresh_panels(s);
     hip_check(hipMemsetAsync(r_.ptr, 0, sizeof(double) * (size_t)A_.n, s), "clear timing input");
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -258,20 +391,19 @@ float Operator::time_spmv(bool panels, hipStream_t s) {
 }
 
 void Operator::set_layout(int layout, hipStream_t s) {
-    bool want;
-    if (layout == kLayoutAuto)
-        want = pan_auto_;
-    else if (layout == kTiles)
-        want = false;
-    else if (layout == kPanels)
-        want = true;
-    else
-        throw invalid("layout must be 0 (auto), 1 (tiles) or 2 (panels).");
-    if (want && !has_pan_ && !build_panels(s))
+    int want = layout == kLayoutAuto ? auto_layout_ : layout;
+    if (want != kTiles && want != kPanels && want != kPatterns)
+        throw invalid("layout must be 0 (auto), 1 (tiles), 2 (panels) or 3 (column-offset patterns).");
+    if (want == kPanels && !has_pan_ && !build_panels(s))
         throw invalid("the panel layout needs each row's columns in ascending order and fewer than 2^26 "
                       "(row block, column panel) segments.");
-    if (want == A_.pan_on) return;
-    A_.pan_on = want;
+    if (want == kPatterns && !has_pat_ && !build_patterns(s))
+        throw invalid("column-offset patterns need every row tile to use at most 32 distinct offsets (col - row), "
+                      "rows of at most 32 entries and strictly ascending columns.");
+    const bool pan = want == kPanels, pat = want == kPatterns;
+    if (pan == A_.pan_on && pat == A_.pat_on) return;
+    A_.pan_on = pan;
+    A_.pat_on = pat;
     drop_graph();
 }
 
@@ -346,7 +478,12 @@ CgParams Operator::cg_params(const double* diag, const double* b, double* x, uin
     P.n = A_.n;
     P.pan_on = A_.pan_on ? 1 : 0;
     P.pan = A_.pan;
-    P.ntiles = A_.pan_on ? A_.pan.nb : A_.ntiles;  // SpMV workgroups = p.Ap partials
+    P.pat_on = A_.pat_on ? 1 : 0;
+    P.pmask = A_.pmask;
+    P.poffs = A_.poffs;
+    P.tiles = A_.ntiles;
+    // p.Ap partials: one per panel row block, per pattern workgroup (two tiles) or per tile
+    P.ntiles = A_.pan_on ? A_.pan.nb : A_.pat_on ? pattern_groups(A_.ntiles) : A_.ntiles;
     P.nnz = (int)A_.nnz;
     P.G2 = G2_;
     P.split_alpha = P.ntiles > kMaxRedundantPartials ? 1 : 0;
@@ -357,6 +494,7 @@ CgParams Operator::cg_params(const double* diag, const double* b, double* x, uin
     P.val = A_.val;
     P.dsplit = A_.dsplit;
     P.tile_row = A_.tile_row;
+    P.tile_k = A_.tile_k;
     P.b = b;
     P.diag = diag;
     P.x = x;
@@ -412,7 +550,8 @@ SolveResult Operator::solve_multi(const double* diag, const double* b, double* x
     // The iteration kernels read b, diag and tol only through the set-up, so a
     // graph captured for (x, records, max_rounds, values) replays for any b.
     if (graph_ok_ && (!gexec_ || gkey_.x != P.x || gkey_.records != P.records || gkey_.max_rounds != P.max_rounds ||
-                      gkey_.val != P.val || gkey_.col != P.col || gkey_.pan_on != P.pan_on))
+                      gkey_.val != P.val || gkey_.col != P.col || gkey_.pan_on != P.pan_on ||
+                      gkey_.pat_on != P.pat_on))
         build_graph(P);
     // Enqueue iterations in chunks; after each chunk the stop flag is copied to
     // pinned memory.  The host checks chunk c-1's flag while chunk c is queued,
@@ -462,6 +601,7 @@ SolveResult Operator::solve_persistent(const double* diag, const double* b, doub
     P.val = A_.val;
     P.dsplit = A_.dsplit;
     P.tile_row = A_.tile_row;
+    P.tile_k = A_.tile_k;
     P.b = b;
     P.diag = diag;
     P.x = x;

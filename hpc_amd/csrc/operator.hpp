@@ -70,6 +70,13 @@ struct PanelPlan {
     std::vector<uint16_t> off, lrow;
     std::vector<int64_t> perm;
 };
+// Host plan of the column-offset pattern tiles (tile_spmv_pattern in
+// kernels.hpp) for a tile plan: false unless every tile is a short-row tile
+// with at most 32 distinct column offsets and every row's columns are
+// strictly ascending and inside [0, n).
+bool plan_patterns(int32_t n, const int32_t* rowptr_host, const int32_t* col_host, const std::vector<int32_t>& tiles,
+                   std::vector<uint32_t>& mask, std::vector<int32_t>& offs);
+
 int panel_block_rows(int32_t n);
 int64_t panel_segments(int32_t n, int block_rows);
 PanelPlan plan_panels(int32_t n, const int32_t* rowptr_host, const int32_t* col_host, int block_rows);
@@ -113,9 +120,11 @@ class Operator {
     // panel-swept copy, for columns scattered over x).  kLayoutAuto restores
     // the choice made at creation: panels when they were measured at least
     // 15% faster on an operator whose columns do not fit the int16 stream.
-    enum Layout { kLayoutAuto = 0, kTiles = 1, kPanels = 2 };
+    // kPatterns runs the row tiles on column-offset patterns (stencil
+    // operators; bit-identical to kTiles), chosen at creation when faster.
+    enum Layout { kLayoutAuto = 0, kTiles = 1, kPanels = 2, kPatterns = 3 };
     void set_layout(int layout, hipStream_t s);
-    int layout() const { return A_.pan_on ? kPanels : kTiles; }
+    int layout() const { return A_.pan_on ? kPanels : A_.pat_on ? kPatterns : kTiles; }
     // use (or stop using) the int16 column stream when it exists
     void set_compact(bool on);
 
@@ -153,10 +162,14 @@ class Operator {
     void build_graph(const CgParams& P);
     void drop_graph();
     // panel layout: build (host plan + upload), release, refresh values, time
-    bool build_panels(hipStream_t s);
+    bool build_panels(hipStream_t s, const int32_t* rp_host = nullptr, const int32_t* col_host = nullptr);
+    bool build_patterns(hipStream_t s, const int32_t* rp_host = nullptr, const int32_t* col_host = nullptr);
+    void copy_pattern_to_host(hipStream_t s, std::vector<int32_t>& rp, std::vector<int32_t>& col) const;
+    float time_layout(int layout, hipStream_t s);
     void free_panels();
+    void free_patterns();
     void refresh_panels(hipStream_t s) const;
-    float time_spmv(bool panels, hipStream_t s);
+
     const void* col_stream() const { return A_.col16 ? (const void*)A_.col16 : (const void*)A_.col; }
 
     static constexpr int kChunk = 8;  // iterations per captured graph
@@ -166,7 +179,7 @@ class Operator {
     static constexpr int64_t kPanelMaxSegments = 1ll << 26;
 
     CsrDev A_;
-    DevBuf<int32_t> tiles_;
+    DevBuf<int32_t> tiles_, tilek_;
     DevBuf<int16_t> col16_;
     bool has_col16_ = false;
     int path_ = kAuto, last_path_ = 0;
@@ -184,8 +197,13 @@ class Operator {
     DevBuf<int64_t> pperm_;
     DevBuf<double> pval_;            // nnz values of the operator's type, in panel order
     bool has_pan_ = false;
-    bool pan_auto_ = false;          // creation-time choice
     mutable bool pan_stale_ = false; // panel values differ from A_.val
+    std::vector<int32_t> tile_plan_;  // host copy of the tile plan
+    DevBuf<uint32_t> pmask_;
+    DevBuf<int32_t> poffs_;
+    bool has_pat_ = false;
+    bool pat_tried_ = false;          // the pattern plan was attempted (and failed if !has_pat_)
+    int auto_layout_ = kTiles;        // creation-time choice
     double* host_result_ = nullptr;  // pinned
     int* host_flags_ = nullptr;      // pinned
     hipStream_t work_ = nullptr;     // graphs are captured and replayed here

@@ -110,10 +110,11 @@ class CsrOperator:
         check_dev(lib().xjpcg_op_set_compact(self._h, -1, ctypes.byref(act)))
         return bool(act.value)
 
-    LAYOUT_AUTO, LAYOUT_TILES, LAYOUT_PANELS = 0, 1, 2
+    LAYOUT_AUTO, LAYOUT_TILES, LAYOUT_PANELS, LAYOUT_PATTERNS = 0, 1, 2, 3
 
     def set_layout(self, layout: int) -> int:
-        """SpMV layout: 0 the creation-time choice, 1 row tiles, 2 panel-swept copy; returns the layout in use."""
+        """SpMV layout: 0 the creation-time choice, 1 row tiles, 2 panel-swept copy, 3 row tiles on column-offset
+        patterns; returns the layout in use."""
         act = ctypes.c_int()
         check_dev(lib().xjpcg_op_set_layout(self._h, int(layout), _stream(), ctypes.byref(act)))
         return act.value
@@ -238,3 +239,36 @@ def panel_plan(rowptr_h, col_h):
                                      lrow.ctypes.data, perm.ctypes.data, ctypes.byref(ordered)))
     return {"block_rows": br.value, "nb": nb.value, "P": P.value, "seg": seg, "off": off, "lrow": lrow,
             "perm": perm, "ordered": bool(ordered.value)}
+
+
+def tile_plan(rowptr_h):
+    """Host-only row tiles of a CSR pattern: first row of each tile (ntiles+1 entries)."""
+    import numpy as np
+    rowptr_h = np.ascontiguousarray(rowptr_h, dtype=np.int32)
+    n = rowptr_h.size - 1
+    tiles = np.zeros(n + 1, np.int32)
+    nt = ctypes.c_int32()
+    check_dev(lib().xjpcg_tile_plan(n, rowptr_h.ctypes.data, tiles.ctypes.data, ctypes.byref(nt)))
+    return tiles[:nt.value + 1].copy()
+
+
+def pattern_plan(rowptr_h, col_h):
+    """Host-only column-offset patterns for the row tiles: (tiles, mask[n] uint32, offs[ntiles, 32]) or None."""
+    import numpy as np
+    rowptr_h = np.ascontiguousarray(rowptr_h, dtype=np.int32)
+    col_h = np.ascontiguousarray(col_h, dtype=np.int32)
+    n = rowptr_h.size - 1
+    tiles = tile_plan(rowptr_h)
+    mask = np.zeros(n, np.uint32)
+    offs = np.zeros((tiles.size - 1, 32), np.int32)
+    ok = ctypes.c_int()
+    check_dev(lib().xjpcg_pattern_plan(n, rowptr_h.ctypes.data, col_h.ctypes.data, mask.ctypes.data,
+                                       offs.ctypes.data, ctypes.byref(ok)))
+    return (tiles, mask, offs) if ok.value else None
+
+
+def stream_copy(src: torch.Tensor, dst: torch.Tensor):
+    """dst = src (fp64, same length) with the library's 16-byte non-temporal stream kernel."""
+    _dev_check(src, torch.float64, "src")
+    _dev_check(dst, torch.float64, "dst", src.numel())
+    check_dev(lib().xjpcg_stream_copy(ptr(src), ptr(dst), src.numel(), _stream()))

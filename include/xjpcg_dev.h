@@ -57,16 +57,21 @@ XJPCG_Status_t xjpcg_op_set_compact(xjpcg_op_t* op, int on, int* h_active);
  * on the stream of its next use. */
 XJPCG_Status_t xjpcg_op_set_values(xjpcg_op_t* op, const void* d_val);
 /* SpMV layout (no counterpart in the reference, whose SpMV streams its own
- * partitioned format from HBM): 1 = row tiles over the CSR arrays;
+ * partitioned format from HBM): 1 = row tiles over the CSR arrays (int16 or
+ * int32 column stream, see xjpcg_op_set_compact);
  * 2 = panel-swept copy for matrices whose columns scatter over x (row blocks
  * of up to 4096 rows keep their sums in LDS and sweep 65536-column panels of
  * x in order, so the gathered window stays in L2; 12 bytes per entry plus an
- * 8-byte value map; rows are still summed left to right).  At creation an
- * operator with at least 2^18 rows whose columns do not fit the int16 stream
- * times both and keeps panels when they are at least 15% faster;
- * layout = 0 restores that choice, layout < 0 only queries.  Layout 2 needs
- * each row's columns in ascending order.  *h_active (may be NULL) receives
- * the layout in use.  Building the panel copy synchronises `stream`. */
+ * 8-byte value map; rows are still summed left to right);
+ * 3 = row tiles on column-offset patterns (stencil operators: each tile's at
+ * most 32 distinct offsets col - row plus a 32-bit mask per row replace the
+ * column stream, 4 bytes per row instead of 2-4 per entry; same sums as 1).
+ * At creation the operator times the layouts that apply and keeps the fastest
+ * (panels only when at least 15% faster than the tiles); layout = 0 restores
+ * that choice, layout < 0 only queries.  Layout 2 needs each row's columns in
+ * ascending order; layout 3 needs strictly ascending columns, rows of at most
+ * 32 entries and at most 32 offsets per tile.  *h_active (may be NULL)
+ * receives the layout in use.  Building a layout synchronises `stream`. */
 XJPCG_Status_t xjpcg_op_set_layout(xjpcg_op_t* op, int layout, void* stream, int* h_active);
 /* Host-only view of the panel layout an operator would build (no device
  * calls): xjpcg_panel_dims gives the rows per block, the row blocks and the
@@ -77,6 +82,14 @@ XJPCG_Status_t xjpcg_op_set_layout(xjpcg_op_t* op, int layout, void* stream, int
 XJPCG_Status_t xjpcg_panel_dims(int32_t n, int32_t* block_rows, int32_t* nb, int32_t* npanels);
 XJPCG_Status_t xjpcg_panel_plan(int32_t n, const int32_t* h_rowptr, const int32_t* h_col, int32_t* seg,
                                 uint16_t* off, uint16_t* lrow, int64_t* perm, int* h_ordered);
+/* Host-only view of the row tiles and their column-offset patterns:
+ * xjpcg_tile_plan writes the first row of each tile (tiles[ntiles+1], at
+ * most n+1 entries) and *h_ntiles; xjpcg_pattern_plan fills mask[n] and
+ * offs[ntiles*32] for that plan and sets *h_ok = 0 when layout 3 does not
+ * apply.  No counterpart in the reference; for inspection and tests. */
+XJPCG_Status_t xjpcg_tile_plan(int32_t n, const int32_t* h_rowptr, int32_t* tiles, int32_t* h_ntiles);
+XJPCG_Status_t xjpcg_pattern_plan(int32_t n, const int32_t* h_rowptr, const int32_t* h_col, uint32_t* mask,
+                                  int32_t* offs, int* h_ok);
 /* n, nnz, number of row tiles, grid of the persistent kernel */
 XJPCG_Status_t xjpcg_op_info(const xjpcg_op_t* op, int32_t* n, int64_t* nnz, int32_t* ntiles, int32_t* grid);
 
@@ -87,6 +100,11 @@ XJPCG_Status_t xjpcg_op_info(const xjpcg_op_t* op, int32_t* n, int64_t* nnz, int
  * runtime that refuses the cooperative launch gets loop 2.  *h_last (may be
  * NULL) receives the loop the last solve used; path < 0 only queries. */
 XJPCG_Status_t xjpcg_op_set_path(xjpcg_op_t* op, int path, int* h_last);
+
+/* d_dst = d_src for n doubles (16-byte aligned), with 16-byte non-temporal
+ * loads and stores: the stream kernel that gives the achievable HBM rate the
+ * SpMV is compared with.  No counterpart in the reference. */
+XJPCG_Status_t xjpcg_stream_copy(const double* d_src, double* d_dst, int64_t n, void* stream);
 
 /* d_y = A d_x  (the reference's SpMV compute units, L2/sparse/src/hw/fp64) */
 XJPCG_Status_t xjpcg_spmv(xjpcg_op_t* op, const double* d_x, double* d_y, void* stream);

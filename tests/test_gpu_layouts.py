@@ -1,5 +1,6 @@
-"""Panel-swept SpMV layout on the GPU: SpMV bit-exact against the oracle (every row, long ones included),
-solve parity, value refresh, split-diagonal fp32 operators."""
+"""SpMV layouts on the GPU.  Panel-swept copy: SpMV bit-exact against the oracle (every row, long ones
+included), solve parity, value refresh, split-diagonal fp32 operators.  Column-offset pattern tiles: SpMV
+bit-exact against the oracle, solves within 1e-12 of the plain row tiles and in parity with the oracle."""
 import sys
 
 import numpy as np
@@ -120,3 +121,63 @@ def test_panel_split_fp32_and_mixed(cuda):
     true = np.linalg.norm(b - A.to_scipy() @ xm.cpu().numpy()) / np.linalg.norm(b)
     _say(f"mixed on panels: outer {info.outer} inner {info.inner_total} true res {true:.2e}")
     assert true < 1e-10
+
+
+@pytest.mark.parametrize("kind", ["2d5", "3d27"])
+def test_pattern_tiles_bitexact_and_identical_solves(cuda, kind):
+    from hpc_amd import device
+    s = workloads.c1_laplace2d(96) if kind == "2d5" else workloads.c2_stencil27(40)
+    A = s.A
+    op = device.CsrOperator(_t(cuda, A.rowptr), _t(cuda, A.col), _t(cuda, A.val), rowptr_host=A.rowptr)
+    _say(f"{kind}: n={A.n} creation-time layout {op.layout}")
+    xv = workloads.uniform(4, 1, A.n, -1, 1)
+    x = _t(cuda, xv)
+    ref = oracle.spmv(A.rowptr, A.col, A.val, xv)
+    diag, b = _t(cuda, A.diag), _t(cuda, s.b)
+    sols = {}
+    for layout in (1, 3):
+        assert op.set_layout(layout) == layout
+        assert np.array_equal(op.spmv(x).cpu().numpy(), ref)
+        r, _ = op.residual(b, x)
+        assert np.array_equal(r.cpu().numpy(), s.b - ref)
+        xs, info = op.solve(diag, b, s.max_iter, s.tol)
+        sols[layout] = (xs.cpu().numpy(), info.executed)
+    # identical SpMV; the solves differ only in how p.Ap partials are grouped (per tile, per pair of tiles)
+    assert abs(sols[1][1] - sols[3][1]) <= 1
+    assert np.linalg.norm(sols[1][0] - sols[3][0]) / np.linalg.norm(sols[1][0]) <= 1e-12
+    o = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, s.b, s.max_iter, s.tol)
+    assert abs(sols[3][1] - o["executed"]) <= 1
+    assert np.linalg.norm(sols[3][0] - o["x"]) / np.linalg.norm(o["x"]) <= 1e-10
+
+
+def test_pattern_tiles_split_fp32(cuda):
+    """fp32 off-diagonal values + fp64 split diagonal: pattern tiles give the tiles' SpMV bit for bit."""
+    from hpc_amd import device
+    A = workloads.c2_stencil27(30).A
+    srp, scol, sv32, sdiag, exact = device.split_fp32(A.rowptr, A.col, A.val)
+    op = device.CsrOperator(_t(cuda, srp), _t(cuda, scol), _t(cuda, sv32), dsplit=_t(cuda, sdiag), rowptr_host=srp)
+    x = _t(cuda, workloads.uniform(4, 2, A.n, -1, 1))
+    assert op.set_layout(1) == 1
+    y1 = op.spmv(x).clone()
+    assert op.set_layout(3) == 3
+    assert bool((op.spmv(x) == y1).all())
+
+
+def test_pattern_layout_refused_for_graphs(cuda):
+    from hpc_amd import device
+    from hpc_amd._lib import XjpcgError
+    A = workloads.graph_laplacian(20000, seed=4)
+    op = device.CsrOperator(_t(cuda, A.rowptr), _t(cuda, A.col), _t(cuda, A.val), rowptr_host=A.rowptr)
+    with pytest.raises(XjpcgError):
+        op.set_layout(3)
+    assert op.layout in (1, 2)
+
+
+def test_stream_copy(cuda):
+    import torch
+    from hpc_amd import device
+    for n in (1, 2, 1001, 1 << 20):
+        src = torch.arange(n, dtype=torch.float64, device=cuda)
+        dst = torch.full_like(src, -1.0)
+        device.stream_copy(src, dst)
+        assert bool((dst == src).all())
