@@ -113,27 +113,30 @@ def cpu_model():
 
 
 def spmv_profile(args):
-    """Short run for hardware-counter passes: a plain 256 MiB copy kernel of known size, then 10 launches
-    of the fp64 SpMV on the default workload (int16 column stream when it applies)."""
+    """Short run for hardware-counter passes: the library's stream kernel copying 256 MiB (the calibration
+    kernel of known size), then 10 launches of the fp64 SpMV in the layout the operator chooses, on the
+    27-point 160^3 operator or (--profile-workload graph) the graph Laplacian."""
     import torch
     from hpc_amd import device, workloads
     dev = torch.device("cuda:0")
-    sysm = workloads.c2_stencil27(args.grid)
-    A = sysm.A
+    if args.profile_workload == "graph":
+        A = workloads.c3_graph(args.graph_n).A
+    else:
+        A = workloads.c2_stencil27(args.grid).A
     op = device.CsrOperator(torch.from_numpy(A.rowptr).to(dev), torch.from_numpy(A.col).to(dev),
                             torch.from_numpy(A.val).to(dev), rowptr_host=A.rowptr)
     src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)
     dst = torch.empty_like(src)
     torch.cuda.synchronize()
-    dst.copy_(src)
+    device.stream_copy(src, dst)
     x = torch.ones(A.n, dtype=torch.float64, device=dev)
     y = torch.empty_like(x)
     for _ in range(10):
         op.spmv(x, y)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_each_way": src.numel() * 8, "spmv_algorithmic_bytes": spmv_bytes(A.n, A.nnz),
-                      "spmv_stored_bytes": (10 if op.compact else 12) * A.nnz + 4 * (A.n + 1) + 16 * A.n,
-                      "compact_columns": op.compact, "n": A.n, "nnz": A.nnz}), flush=True)
+                      "layout": LAYOUT_NAMES[op.layout], "compact_columns": op.compact, "n": A.n, "nnz": A.nnz}),
+          flush=True)
 
 
 def _events(stream):
@@ -385,6 +388,7 @@ def main():
                          "for rocprofv3 --pmc runs; graph / batched / autograd: the graph-Laplacian, batched "
                          "and differentiable workloads, one JSON line each")
     ap.add_argument("--graph-n", type=int, default=4_000_000)
+    ap.add_argument("--profile-workload", choices=["stencil27", "graph"], default="stencil27")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--nrhs", type=int, default=16)
     args = ap.parse_args()
