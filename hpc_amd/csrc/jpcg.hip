@@ -90,9 +90,8 @@ __global__ __launch_bounds__(kBlock) void jpcg_persistent(JpcgParams P) {
             for (int t = sw.first; t < sw.end; t += sw.step) {
                 double y;
                 int row;
-                tile_spmv<VT, CT>(P.tile_row[t], P.tile_row[t + 1], P.tile_k[t], P.tile_k[t + 1], P.nnz, P.rowptr,
-                                  col, val, P.dsplit, p, lds,
-                                  red, &y, &row);
+                tile_spmv<VT, CT>(t, P.tile_row, P.tile_k, P.nnz, P.rowptr, col, val, P.dsplit, p, lds, red, &y,
+                                  &row);
                 if (row >= 0) {
                     Ap[row] = y;
                     pap += p[row] * y;
@@ -222,63 +221,91 @@ __global__ __launch_bounds__(kBlock) void cg_setup_finish(CgParams P) {
     }
 }
 
+// CG-loop SpMV on row tiles: logical workgroup w runs tiles kRowTiles*w ..
+// (consecutive rows), the p.Ap partial of tile t in partA[t] (the same
+// partials, in the same order, whatever the layout).  The stop flag, the
+// tiles' loads and this lane's p[row] are all issued before the workgroup
+// waits on any of them.
 template <typename VT, typename CT>
 __global__ __launch_bounds__(kBlock) void cg_spmv(CgParams P) {
     __shared__ double lds[kTileNnz];
     __shared__ double red[kWaves];
     CgState* st = P.st;
-    if (st->stop) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->k = st->k + 1;  // no other workgroup of this launch reads k
-    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int stop = st->stop;
+    const int w = xcd_remap(blockIdx.x, gridDim.x);
     const double* __restrict__ p = P.p;
-    double y;
-    int row;
-    tile_spmv<VT, CT>(P.tile_row[t], P.tile_row[t + 1], P.tile_k[t], P.tile_k[t + 1], P.nnz, P.rowptr,
-                      static_cast<const CT*>(P.col),
-                      static_cast<const VT*>(P.val), P.dsplit, p, lds, red, &y, &row);
-    double pap = 0.0;
-    if (row >= 0) {
-        P.Ap[row] = y;
-        pap = p[row] * y;
+    const CT* __restrict__ col = static_cast<const CT*>(P.col);
+    const VT* __restrict__ val = static_cast<const VT*>(P.val);
+    RowTile<VT, CT> T[kRowTiles];
+    double prow[kRowTiles];
+#pragma unroll
+    for (int u = 0; u < kRowTiles; ++u) {
+        const int t = w * kRowTiles + u;
+        prow[u] = 0.0;
+        if (t < P.tiles) {
+            row_tile_load<VT, CT>(T[u], t, P.tile_row, P.tile_k, P.nnz, P.rowptr, col, val);
+            if ((int)threadIdx.x < T[u].r1 - T[u].r0) prow[u] = p[T[u].r0 + threadIdx.x];
+        }
     }
-    pap = block_sum(pap, red);
-    if (threadIdx.x == 0) P.partA[t] = pap;
+    if (stop) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->k = st->k + 1;  // no other workgroup of this launch reads k
+#pragma unroll
+    for (int u = 0; u < kRowTiles; ++u) {
+        if (T[u].t < 0) break;
+        double y;
+        int row;
+        row_tile_compute<VT, CT>(T[u], P.rowptr, col, val, P.dsplit, p, lds, red, &y, &row);
+        double pap = 0.0;
+        if (row >= 0) {
+            P.Ap[row] = y;
+            pap = prow[u] * y;
+        }
+        pap = block_sum(pap, red);
+        if (threadIdx.x == 0) P.partA[T[u].t] = pap;
+    }
 }
 
 // CG-loop SpMV on column-offset pattern tiles: logical workgroup w runs tiles
-// 2w and 2w+1 (consecutive rows); the p.Ap partial of both is partA[w]
+// 2w and 2w+1 (consecutive rows); the p.Ap partial of tile t is partA[t]
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void cg_spmv_pattern(CgParams P) {
     __shared__ double lds[kTileNnz];
     __shared__ double red[kWaves];
     __shared__ int loffs[32];
     CgState* st = P.st;
-    if (st->stop) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->k = st->k + 1;  // no other workgroup of this launch reads k
+    // the stop flag, the tiles' loads and this lane's p[row] are all issued
+    // before the workgroup waits on any of them
+    const int stop = st->stop;
     const int w = xcd_remap(blockIdx.x, gridDim.x);
     const double* __restrict__ p = P.p;
     const VT* __restrict__ val = static_cast<const VT*>(P.val);
     PatternTile<VT> T[kPatternTiles];
+    double prow[kPatternTiles];
 #pragma unroll
     for (int u = 0; u < kPatternTiles; ++u) {
         const int t = w * kPatternTiles + u;
-        if (t < P.tiles)
+        prow[u] = 0.0;
+        if (t < P.tiles) {
             pattern_tile_load<VT>(T[u], t, P.tile_row, P.tile_k, P.nnz, P.rowptr, P.pmask, P.poffs, val);
+            if ((int)threadIdx.x < T[u].r1 - T[u].r0) prow[u] = p[T[u].r0 + threadIdx.x];
+        }
     }
-    double pap = 0.0;
+    if (stop) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->k = st->k + 1;  // no other workgroup of this launch reads k
 #pragma unroll
     for (int u = 0; u < kPatternTiles; ++u) {
         if (T[u].t < 0) break;
         double y;
         int row;
         pattern_tile_compute<VT>(T[u], P.dsplit, p, lds, loffs, &y, &row);
+        double pap = 0.0;
         if (row >= 0) {
             P.Ap[row] = y;
-            pap += p[row] * y;
+            pap = prow[u] * y;
         }
+        pap = block_sum(pap, red);
+        if (threadIdx.x == 0) P.partA[T[u].t] = pap;
     }
-    pap = block_sum(pap, red);
-    if (threadIdx.x == 0) P.partA[w] = pap;
 }
 
 // second reduction level for many tiles: workgroup b sums a contiguous slice
@@ -375,13 +402,21 @@ __global__ __launch_bounds__(kBlock) void spmv_tiles(int ntiles, int nnz, const 
                                                      const double* __restrict__ b) {
     __shared__ double lds[kTileNnz];
     __shared__ double red[kWaves];
-    const int t = xcd_remap(blockIdx.x, gridDim.x);
-    if (t >= ntiles) return;
-    double v;
-    int row;
-    tile_spmv<VT, CT>(tile_row[t], tile_row[t + 1], tile_k[t], tile_k[t + 1], nnz, rowptr, col, val, dsplit, x, lds,
-                      red, &v, &row);
-    if (row >= 0) y[row] = b ? b[row] - v : v;
+    const int w = xcd_remap(blockIdx.x, gridDim.x);
+    RowTile<VT, CT> T[kRowTiles];
+#pragma unroll
+    for (int u = 0; u < kRowTiles; ++u) {
+        const int t = w * kRowTiles + u;
+        if (t < ntiles) row_tile_load<VT, CT>(T[u], t, tile_row, tile_k, nnz, rowptr, col, val);
+    }
+#pragma unroll
+    for (int u = 0; u < kRowTiles; ++u) {
+        if (T[u].t < 0) break;
+        double v;
+        int row;
+        row_tile_compute<VT, CT>(T[u], rowptr, col, val, dsplit, x, lds, red, &v, &row);
+        if (row >= 0) y[row] = b ? b[row] - v : v;
+    }
 }
 
 // --------------------------------------------------------------------------
@@ -677,7 +712,7 @@ hipError_t launch_cg_iteration(const CgParams& P, bool f32, bool c16, hipStream_
         else
             cg_spmv_pattern<double><<<pattern_groups(P.tiles), kBlock, 0, s>>>(P);
     } else {
-        XJ_DISPATCH_VC(f32, c16, (cg_spmv<VT, CT><<<P.ntiles, kBlock, 0, s>>>(P)));
+        XJ_DISPATCH_VC(f32, c16, (cg_spmv<VT, CT><<<tile_groups(P.tiles), kBlock, 0, s>>>(P)));
     }
     if (P.split_alpha) cg_alpha<<<P.nA2, kBlock, 0, s>>>(P);
     cg_update_r<<<P.G2, kBlock, 0, s>>>(P);
@@ -709,7 +744,8 @@ hipError_t launch_spmv(const CsrDev& A, const double* x, double* y, const double
     }
     const void* col = A.col16 ? (const void*)A.col16 : (const void*)A.col;
     XJ_DISPATCH_VC(A.val_f32, A.col16 != nullptr,
-                   (spmv_tiles<VT, CT><<<A.ntiles, kBlock, 0, s>>>(A.ntiles, (int)A.nnz, A.tile_row, A.tile_k,
+                   (spmv_tiles<VT, CT><<<tile_groups(A.ntiles), kBlock, 0, s>>>(A.ntiles, (int)A.nnz, A.tile_row,
+                                                                                A.tile_k,
                                                                    A.rowptr,
                                                                    static_cast<const CT*>(col),
                                                                    static_cast<const VT*>(A.val), A.dsplit, x, y,

@@ -153,24 +153,34 @@ __device__ __forceinline__ void load_run(const T* __restrict__ p, T (&out)[E]) {
     __builtin_memcpy(out, &w, sizeof(W));
 }
 
-// k0 = rowptr[r0], k1 = rowptr[r1] come from the per-tile table tile_k, so
-// they load together with r0 and r1 instead of after them.
+// What one lane needs of a row tile before its compute starts, loaded up
+// front so a workgroup can have a later tile's loads (and the CG loop its
+// stop flag) in flight together.  k0 = rowptr[r0] and k1 = rowptr[r1] come
+// from the per-tile table tile_k, so they load together with r0 and r1.
 template <typename VT, typename CT>
-__device__ __forceinline__ void tile_spmv(int r0, int r1, int k0, int k1, int nnz, const int* __restrict__ rowptr,
-                                          const CT* __restrict__ col, const VT* __restrict__ val,
-                                          const double* __restrict__ dsplit, const double* __restrict__ x,
-                                          double* lds, double* red, double* y, int* row) {
-#pragma clang fp contract(off)
+struct RowTile {
+    static constexpr int E = 16 / (int)sizeof(VT);
+    static constexpr int kGroups = kTileNnz / (kBlock * E);
+    int t = -1, r0 = 0, r1 = 0, k0 = 0, k1 = 0;
+    VT v[kGroups][E];
+    CT c[kGroups][E];
+    __device__ bool long_row() const { return k1 - k0 > kTilePlanNnz; }
+};
+
+template <typename VT, typename CT>
+__device__ __forceinline__ void row_tile_load(RowTile<VT, CT>& T, int t, const int* __restrict__ tile_row,
+                                              const int* __restrict__ tile_k, int nnz,
+                                              const int* __restrict__ rowptr, const CT* __restrict__ col,
+                                              const VT* __restrict__ val) {
+    constexpr int E = RowTile<VT, CT>::E, kGroups = RowTile<VT, CT>::kGroups;
     const int tid = threadIdx.x;
-    if (k1 - k0 > kTilePlanNnz) {
-        double s = 0.0;
-        for (int k = k0 + tid; k < k1; k += kBlock) s += (double)ld_stream(val + k) * x[col_at(col, k, r0)];
-        s = block_sum(s, red);
-        if (dsplit) s = dsplit[r0] * x[r0] + s;
-        *y = s;
-        *row = (tid == 0) ? r0 : -1;
-        return;
-    }
+    T.t = t;
+    T.r0 = tile_row[t];
+    T.r1 = tile_row[t + 1];
+    T.k0 = tile_k[t];
+    T.k1 = tile_k[t + 1];
+    const int k0 = T.k0, k1 = T.k1;
+    if (k1 - k0 > kTilePlanNnz) return;  // one long row: its compute streams it
     if (k1 > k0) {
         // Each lane loads runs of E = 16/sizeof(VT) consecutive entries (one
         // 16-byte value load, one matching column load), kGroups runs spaced a
@@ -181,20 +191,14 @@ __device__ __forceinline__ void tile_spmv(int r0, int r1, int k0, int k1, int nn
         // the arrays unless the tile ends in a final partial run of the whole
         // array (nnz not a multiple of E); only that tile loads entry by
         // entry.  Lanes with nothing to load re-read the tile's first run.
-        // All loads of a lane issue before its gathers, and all gathers
-        // before the products, so their latencies overlap.
-        constexpr int E = 16 / (int)sizeof(VT);
-        constexpr int kGroups = kTileNnz / (kBlock * E);
         const int k0a = k0 & ~(E - 1);
-        VT v[kGroups][E];
-        CT c[kGroups][E];
         if (((k1 + E - 1) & ~(E - 1)) <= nnz) {
 #pragma unroll
             for (int g = 0; g < kGroups; ++g) {
                 int base = k0a + (g * kBlock + tid) * E;
                 if (base >= k1) base = k0a;
-                load_run<VT, E>(val + base, v[g]);
-                load_run<CT, E>(col + base, c[g]);
+                load_run<VT, E>(val + base, T.v[g]);
+                load_run<CT, E>(col + base, T.c[g]);
             }
         } else {
 #pragma unroll
@@ -203,21 +207,50 @@ __device__ __forceinline__ void tile_spmv(int r0, int r1, int k0, int k1, int nn
                 for (int e = 0; e < E; ++e) {
                     const int kk = k0a + (g * kBlock + tid) * E + e;
                     const int k = kk >= k0 && kk < k1 ? kk : k0;
-                    v[g][e] = ld_stream(val + k);
-                    c[g][e] = ld_stream(col + k);
+                    T.v[g][e] = ld_stream(val + k);
+                    T.c[g][e] = ld_stream(col + k);
                 }
             }
         }
+    }
+}
+
+// Compute of a loaded row tile: gathers (all of a lane's in flight
+// together), products in LDS, lane-per-row left-to-right sums; a long row is
+// summed by the whole workgroup with a strided loop and a fixed-order
+// workgroup reduction.  On return lanes with *row >= 0 hold y[*row] in *y;
+// ends with a __syncthreads so the workgroup's next tile can reuse lds / red.
+template <typename VT, typename CT>
+__device__ __forceinline__ void row_tile_compute(const RowTile<VT, CT>& T, const int* __restrict__ rowptr,
+                                                 const CT* __restrict__ col,
+                                                 const VT* __restrict__ val, const double* __restrict__ dsplit,
+                                                 const double* __restrict__ x, double* lds, double* red, double* y,
+                                                 int* row) {
+#pragma clang fp contract(off)
+    constexpr int E = RowTile<VT, CT>::E, kGroups = RowTile<VT, CT>::kGroups;
+    const int tid = threadIdx.x;
+    const int r0 = T.r0, r1 = T.r1, k0 = T.k0, k1 = T.k1;
+    if (T.long_row()) {
+        double s = 0.0;
+        for (int k = k0 + tid; k < k1; k += kBlock) s += (double)ld_stream(val + k) * x[col_at(col, k, r0)];
+        s = block_sum(s, red);
+        if (dsplit) s = dsplit[r0] * x[r0] + s;
+        *y = s;
+        *row = (tid == 0) ? r0 : -1;
+        return;
+    }
+    if (k1 > k0) {
         // Entries outside [k0, k1) belong to neighbouring tiles: their int16
         // deltas are relative to another tile's first row, so decoding them
         // here could index past x.  They gather x[r0] instead (and are dropped).
+        const int k0a = k0 & ~(E - 1);
         double gx[kGroups][E];
 #pragma unroll
         for (int g = 0; g < kGroups; ++g)
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int k = k0a + (g * kBlock + tid) * E + e;
-                gx[g][e] = x[(k >= k0 && k < k1) ? col_of(c[g][e], r0) : r0];
+                gx[g][e] = x[(k >= k0 && k < k1) ? col_of(T.c[g][e], r0) : r0];
             }
 #pragma unroll
         for (int g = 0; g < kGroups; ++g) {
@@ -225,12 +258,14 @@ __device__ __forceinline__ void tile_spmv(int r0, int r1, int k0, int k1, int nn
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int k = base + e;
-                if (k >= k0 && k < k1) lds[k - k0] = (double)v[g][e] * gx[g][e];
+                if (k >= k0 && k < k1) lds[k - k0] = (double)T.v[g][e] * gx[g][e];
             }
         }
     }
     __syncthreads();
     if (tid < r1 - r0) {
+        // a lane's row bounds load here, not with the tile: holding them across
+        // the gathers costs more (measured) than the load
         const int a = rowptr[r0 + tid] - k0, b = rowptr[r0 + tid + 1] - k0;
         double s = dsplit ? dsplit[r0 + tid] * x[r0 + tid] : 0.0;
         // left-to-right sum; the LDS reads of a group of four issue together
@@ -250,6 +285,22 @@ __device__ __forceinline__ void tile_spmv(int r0, int r1, int k0, int k1, int nn
     }
     __syncthreads();
 }
+
+// One row tile, loaded and computed (the persistent kernel's tile loop).
+template <typename VT, typename CT>
+__device__ __forceinline__ void tile_spmv(int t, const int* __restrict__ tile_row, const int* __restrict__ tile_k,
+                                          int nnz, const int* __restrict__ rowptr, const CT* __restrict__ col,
+                                          const VT* __restrict__ val, const double* __restrict__ dsplit,
+                                          const double* __restrict__ x, double* lds, double* red, double* y,
+                                          int* row) {
+    RowTile<VT, CT> T;
+    row_tile_load<VT, CT>(T, t, tile_row, tile_k, nnz, rowptr, col, val);
+    row_tile_compute<VT, CT>(T, rowptr, col, val, dsplit, x, lds, red, y, row);
+}
+
+// Row tiles per workgroup in the standalone and CG-loop SpMV kernels (with
+// more than one, later tiles' loads are issued before the first compute).
+constexpr int kRowTiles = kRowTilesHost;
 
 // Column-offset pattern tiles.  When every tile of the plan uses at most 32
 // distinct column offsets (col - row), as stencil operators do, the operator

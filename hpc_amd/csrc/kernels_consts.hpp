@@ -28,4 +28,8 @@ constexpr int panel_lds_bytes(int block_rows) { return block_rows * 8 + kPanelCh
 // one p.Ap partial per workgroup
 constexpr int kPatternTilesHost = 2;
 constexpr int pattern_groups(int ntiles) { return (ntiles + kPatternTilesHost - 1) / kPatternTilesHost; }
+// tiles per workgroup of the row-tile SpMV kernels (measured on the 160^3
+// stencil: 1 beats 2, which costs registers — 238 vs 260 us for fp64 values)
+constexpr int kRowTilesHost = 1;
+constexpr int tile_groups(int ntiles) { return (ntiles + kRowTilesHost - 1) / kRowTilesHost; }
 }  // namespace xj

@@ -482,8 +482,8 @@ CgParams Operator::cg_params(const double* diag, const double* b, double* x, uin
     P.pmask = A_.pmask;
     P.poffs = A_.poffs;
     P.tiles = A_.ntiles;
-    // p.Ap partials: one per panel row block, per pattern workgroup (two tiles) or per tile
-    P.ntiles = A_.pan_on ? A_.pan.nb : A_.pat_on ? pattern_groups(A_.ntiles) : A_.ntiles;
+    // p.Ap partials: one per panel row block, or one per row tile
+    P.ntiles = A_.pan_on ? A_.pan.nb : A_.ntiles;
     P.nnz = (int)A_.nnz;
     P.G2 = G2_;
     P.split_alpha = P.ntiles > kMaxRedundantPartials ? 1 : 0;
