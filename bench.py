@@ -313,6 +313,43 @@ def bench_batched(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_ingest(args):
+    """One-shot pcg.h solve from a shuffled COO of the 27-point operator (--grid, default 160^3): the
+    reference's metric phases (matrix processing, vectors, solver) with the CSR built on the GPU, beside the
+    oracle's host COO-to-CSR on the same arrays."""
+    import torch
+    from hpc_amd import workloads
+    from hpc_amd.pcg import JPCG
+    from oracle import oracle
+    sysm = workloads.c2_stencil27(args.grid)
+    A = sysm.A
+    rng = np.random.default_rng(5)
+    p = rng.permutation(A.nnz)
+    rows = np.repeat(np.arange(A.n, dtype=np.uint32), np.diff(A.rowptr))[p]
+    cols = A.col.astype(np.uint32)[p]
+    vals = A.val[p]
+    del p
+    log(f"COO of n={A.n}, nnz={A.nnz} shuffled")
+    out = {}
+    with JPCG(0) as h:
+        for rep in range(2):
+            t0 = time.perf_counter()
+            res = h.coo_solve(A.n, rows, cols, vals, A.diag, sysm.b, sysm.max_iter, sysm.tol)
+            wall = time.perf_counter() - t0
+            m = h.metrics()
+            out[f"call{rep}"] = {"wall_s": round(wall, 3), "matProc_s": round(m.m_matProc, 3),
+                                 "vecProc_s": round(m.m_vecProc, 3), "solver_s": round(m.m_solver, 3),
+                                 "iterations": res.iterations, "status": int(res.status)}
+            log(f"pcg.h call {rep}: {out[f'call{rep}']}")
+    t0 = time.perf_counter()
+    oracle.coo_to_csr(A.n, rows, cols, vals)
+    cpu = time.perf_counter() - t0
+    log(f"oracle host COO->CSR (1 thread): {cpu:.2f} s")
+    print(json.dumps({"metric": "one-shot xJPCG_cooSolver phases, CSR built on the GPU",
+                      "workload": f"27-point stencil {args.grid}^3 as a shuffled COO (n={A.n}, nnz={A.nnz})",
+                      **out, "oracle_coo_to_csr_s": round(cpu, 3)}), flush=True)
+
+
 def batched_profile(args):
     """Short run for hardware-counter passes on the batched kernels: the batch of --batch-count systems
     x --nrhs right-hand sides, 10 fp64 iterations (set-up plus 10 graph-free iteration launches)."""
@@ -382,7 +419,8 @@ def main():
     ap.add_argument("--spmv-repeats", type=int, default=100)
     ap.add_argument("--inner-tol", type=float, default=0.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", choices=["full", "spmv-profile", "graph", "batched", "autograd", "batched-profile"],
+    ap.add_argument("--mode", choices=["full", "spmv-profile", "graph", "batched", "autograd", "batched-profile",
+                                       "ingest"],
                     default="full",
                     help="spmv-profile: one 256 MiB device copy (counter calibration) then 10 SpMV launches, "
                          "for rocprofv3 --pmc runs; graph / batched / autograd: the graph-Laplacian, batched "
@@ -399,12 +437,12 @@ def main():
     if args.mode == "spmv-profile":
         spmv_profile(args)
         return
-    if args.mode in ("graph", "batched", "autograd", "batched-profile"):
+    if args.mode in ("graph", "batched", "autograd", "batched-profile", "ingest"):
         import torch
         assert torch.cuda.is_available(), "bench.py needs a GPU"
         torch.cuda.set_device(0)
         {"graph": bench_graph, "batched": bench_batched, "autograd": bench_autograd,
-         "batched-profile": batched_profile}[args.mode](args)
+         "batched-profile": batched_profile, "ingest": bench_ingest}[args.mode](args)
         return
 
     import torch

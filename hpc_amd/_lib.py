@@ -46,6 +46,10 @@ _SIGNATURES = {
                                  POINTER(c_int)]),
     "xjpcg_tile_plan": (c_int, [c_int32, c_void_p, c_void_p, POINTER(c_int32)]),
     "xjpcg_stream_copy": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
+    "xjpcg_csr_from_coo": (c_int, [c_uint32, c_uint32, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                   c_void_p]),
+    "xjpcg_csr_from_cscsym": (c_int, [c_uint32, c_uint32, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                      POINTER(c_int64), c_void_p]),
     "xjpcg_pattern_plan": (c_int, [c_int32, c_void_p, c_void_p, c_void_p, c_void_p, POINTER(c_int)]),
     "xjpcg_spmv": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "xjpcg_residual": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, POINTER(c_double), c_void_p]),

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <string>
 
+#include "ingest.hpp"
 #include "operator.hpp"
 
 struct xjpcg_op {
@@ -151,6 +152,28 @@ XJPCG_Status_t xjpcg_op_set_path(xjpcg_op_t* op, int path, int* h_last) {
         if (path < 0) return;  // query only
         if (path > 2) throw xj::invalid("path must be 0, 1 or 2.");
         o.set_path(path);
+    });
+}
+
+XJPCG_Status_t xjpcg_csr_from_coo(uint32_t n, uint32_t nnz, const uint32_t* h_row, const uint32_t* h_col, int base,
+                                  int32_t* d_rowptr, int32_t* d_col, int64_t* d_perm, void* stream) {
+    return guarded([&] {
+        if (n == 0) throw xj::invalid("Wrong dimension size.");
+        if (!d_rowptr || (nnz > 0 && (!h_row || !h_col || !d_col || !d_perm))) throw xj::invalid("Matrix is nullptr.");
+        xj::csr_from_coo_device(n, nnz, h_row, h_col, base, d_rowptr, d_col, d_perm, as_stream(stream));
+    });
+}
+
+XJPCG_Status_t xjpcg_csr_from_cscsym(uint32_t n, uint32_t nnz_half, const int64_t* h_row_idx,
+                                     const int64_t* h_col_ptr, int base, int32_t* d_rowptr, int32_t* d_col,
+                                     int64_t* d_perm, int64_t* h_nnz, void* stream) {
+    return guarded([&] {
+        if (n == 0) throw xj::invalid("Wrong dimension size.");
+        if (!h_col_ptr || !d_rowptr || (nnz_half > 0 && (!h_row_idx || !d_col || !d_perm)))
+            throw xj::invalid("Matrix is nullptr.");
+        const uint32_t full = xj::csr_from_cscsym_device(n, nnz_half, h_row_idx, h_col_ptr, base, d_rowptr, d_col,
+                                                         d_perm, as_stream(stream));
+        if (h_nnz) *h_nnz = full;
     });
 }
 

@@ -16,6 +16,7 @@
 #include <thread>
 #include <vector>
 
+#include "ingest.hpp"
 #include "operator.hpp"
 #include "pcg.h"
 #include "xjpcg_ext.h"
@@ -30,134 +31,6 @@ double elapsed(Clock::time_point& last) {
     double d = std::chrono::duration<double>(now - last).count();
     last = now;
     return d;
-}
-
-// Errors the reference's partitioner raises as SpmException (not a
-// CgException), which pcg.cpp:112-113 maps to XJPCG_STATUS_INTERNAL_ERROR.
-Error spm_invalid(const std::string& m) { return Error(XJPCG_STATUS_INTERNAL_ERROR, "INVALID VALUE ERROR: " + m); }
-Error spm_internal(const std::string& m) { return Error(XJPCG_STATUS_INTERNAL_ERROR, "INTERNAL ERROR: " + m); }
-Error spm_not_supported(const std::string& m) {
-    return Error(XJPCG_STATUS_INTERNAL_ERROR, "Matrix Operation ERROR: " + m);
-}
-
-// Host CSR pattern with a map from CSR slot to the caller's value array.
-struct Pattern {
-    uint32_t n = 0;
-    int64_t nnz = 0;       // stored entries after expansion
-    int64_t src_len = 0;   // length of the caller's value array
-    std::vector<int32_t> rowptr, col;
-    std::vector<int64_t> perm;
-};
-
-// Bucket (row, col, src) triples by row, stably, then order each row by
-// column (ties keep input order, so duplicates stay adjacent and are summed
-// by the SpMV like the reference accumulates them).
-void finish_pattern(Pattern& P, const std::vector<uint32_t>& r, const std::vector<uint32_t>& c,
-                    const std::vector<int64_t>& src) {
-    const uint32_t n = P.n;
-    const int64_t m = (int64_t)r.size();
-    P.nnz = m;
-    P.rowptr.assign((size_t)n + 1, 0);
-    for (int64_t k = 0; k < m; ++k) P.rowptr[r[k] + 1]++;
-    for (uint32_t i = 0; i < n; ++i) P.rowptr[i + 1] += P.rowptr[i];
-    P.col.resize(m);
-    P.perm.resize(m);
-    {
-        std::vector<int64_t> pos(P.rowptr.begin(), P.rowptr.end() - 1);
-        for (int64_t k = 0; k < m; ++k) {
-            int64_t p = pos[r[k]]++;
-            P.col[p] = (int32_t)c[k];
-            P.perm[p] = src[k];
-        }
-    }
-    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    if (m < (1 << 20)) nt = 1;
-    std::vector<std::thread> th;
-    for (unsigned t = 0; t < nt; ++t) {
-        th.emplace_back([&, t] {
-            std::vector<std::pair<int32_t, int64_t>> tmp;
-            for (uint32_t i = (uint32_t)((uint64_t)n * t / nt); i < (uint32_t)((uint64_t)n * (t + 1) / nt); ++i) {
-                const int32_t a = P.rowptr[i], b = P.rowptr[i + 1];
-                bool sorted = true;
-                for (int32_t k = a + 1; k < b; ++k)
-                    if (P.col[k] < P.col[k - 1]) { sorted = false; break; }
-                if (sorted) continue;
-                tmp.clear();
-                for (int32_t k = a; k < b; ++k) tmp.emplace_back(P.col[k], P.perm[k]);
-                std::stable_sort(tmp.begin(), tmp.end(),
-                                 [](const std::pair<int32_t, int64_t>& x, const std::pair<int32_t, int64_t>& y) {
-                                     return x.first < y.first;
-                                 });
-                for (int32_t k = a; k < b; ++k) {
-                    P.col[k] = tmp[k - a].first;
-                    P.perm[k] = tmp[k - a].second;
-                }
-            }
-        });
-    }
-    for (auto& x : th) x.join();
-}
-
-// COO with index base (reference matrix_params.hpp:51-79 loadCoo)
-void pattern_from_coo(Pattern& P, uint32_t n, uint32_t nnz, const uint32_t* row, const uint32_t* col, int base) {
-    if (base != 0 && base != 1)
-        throw spm_not_supported("from getCooDatFromCscSym in gen_signature.hpp, unsupported storage mode.");
-    if ((int64_t)nnz >= (int64_t)INT32_MAX) throw invalid("number of non-zeros must be below 2^31.");
-    const uint32_t off = (uint32_t)base;
-    std::vector<uint32_t> r(nnz), c(nnz);
-    std::vector<int64_t> src(nnz);
-    for (uint32_t k = 0; k < nnz; ++k) {
-        uint32_t i = row[k] - off, j = col[k] - off;
-        if (i >= n || j >= n) throw invalid("matrix index out of range at entry " + std::to_string(k) + ".");
-        r[k] = i;
-        c[k] = j;
-        src[k] = k;
-    }
-    P.n = n;
-    P.src_len = nnz;
-    finish_pattern(P, r, c, src);
-}
-
-// Half-stored symmetric CSC (reference matrix_params.hpp:81-134 loadCscSym,
-// gen_signature.hpp:88-112 getCooDatFromCscSym): each off-diagonal entry is
-// emitted twice with the same value, the expansion must produce exactly
-// 2*nnz_half - n entries.
-void pattern_from_cscsym(Pattern& P, uint32_t n, uint32_t nnz_half, const int64_t* row_idx, const int64_t* col_ptr,
-                         int base) {
-    if (base != 0 && base != 1)
-        throw spm_not_supported("from getCooDatFromCscSym in gen_signature.hpp, unsupported storage mode.");
-    const int64_t off = base;
-    if (base == 1 && col_ptr[0] < 1)
-        throw spm_invalid("from getCooDatFromCscSym in gen_signature.hpp, colPtr[0] start from 1 in FORTRAN storage mode.");
-    const uint32_t full = nnz_half * 2u - n;  // uint32 arithmetic as pcgImp.hpp:87
-    if ((int64_t)full >= (int64_t)INT32_MAX) throw invalid("number of non-zeros must be below 2^31.");
-    std::vector<uint32_t> r, c;
-    std::vector<int64_t> src;
-    r.reserve(full);
-    c.reserve(full);
-    src.reserve(full);
-    for (uint32_t j = 0; j < n; ++j) {
-        for (int64_t k = col_ptr[j] - off; k < col_ptr[j + 1] - off; ++k) {
-            if (k < 0 || k >= (int64_t)nnz_half) throw spm_internal("from loadCscSym in matrix_params.hpp, index >= nnz.");
-            int64_t i = row_idx[k] - off;
-            if (i < 0 || i >= (int64_t)n) throw invalid("matrix index out of range at entry " + std::to_string(k) + ".");
-            if (r.size() >= full) throw spm_internal("from loadCscSym in matrix_params.hpp, index >= nnz.");
-            r.push_back((uint32_t)i);
-            c.push_back(j);
-            src.push_back(k);
-            if (i != (int64_t)j) {
-                if (r.size() >= full) throw spm_internal("from loadCscSym in matrix_params.hpp, index >= nnz.");
-                r.push_back(j);
-                c.push_back((uint32_t)i);
-                src.push_back(k);
-            }
-        }
-    }
-    if (r.size() != full)
-        throw spm_internal("from loadCscSym in matrix_params.hpp, failed to convert indices from CscSym to COO format.");
-    P.n = n;
-    P.src_len = nnz_half;
-    finish_pattern(P, r, c, src);
 }
 
 }  // namespace
@@ -211,10 +84,18 @@ class PcgHandle {
         if (n == 0) throw invalid("Wrong dimension size.");
         if (nnz == 0) throw invalid("Wrong non-zero element size.");
         if (row == nullptr || col == nullptr || data == nullptr) throw invalid("Matrix is nullptr.");
-        Pattern P;
-        pattern_from_coo(P, n, nnz, row, col, base);
+        if (nnz < kDeviceIngestMin) {
+            Pattern P;
+            pattern_from_coo(P, n, nnz, row, col, base);
+            require_device();
+            upload_pattern(std::move(P), data);
+            return;
+        }
+        validate_coo(n, nnz, row, col, base);  // every error before the device is touched
         require_device();
-        upload_pattern(std::move(P), data);
+        build_on_device(n, nnz, nnz, data, [&](int32_t* rp, int32_t* c, int64_t* pm) {
+            csr_from_coo_device(n, nnz, row, col, base, rp, c, pm, stream_);
+        });
     }
 
     // reference PCGImpl::updateMat (pcgImp.hpp:95-116)
@@ -234,10 +115,18 @@ class PcgHandle {
         if (n == 0) throw invalid("Wrong dimension size.");
         if (nnz_half == 0) throw invalid("Wrong non-zero element size.");
         if (row_idx == nullptr || col_ptr == nullptr || data == nullptr) throw invalid("Matrix is nullptr.");
-        Pattern P;
-        pattern_from_cscsym(P, n, nnz_half, row_idx, col_ptr, base);
+        if (nnz_half < kDeviceIngestMin) {
+            Pattern P;
+            pattern_from_cscsym(P, n, nnz_half, row_idx, col_ptr, base);
+            require_device();
+            upload_pattern(std::move(P), data);
+            return;
+        }
+        const uint32_t full = validate_cscsym(n, nnz_half, row_idx, col_ptr, base, nullptr);
         require_device();
-        upload_pattern(std::move(P), data);
+        build_on_device(n, full, nnz_half, data, [&](int32_t* rp, int32_t* c, int64_t* pm) {
+            csr_from_cscsym_device(n, nnz_half, row_idx, col_ptr, base, rp, c, pm, stream_);
+        });
     }
 
     // reference PCGImpl::updateCscSymMat (pcgImp.hpp:117-145)
@@ -299,6 +188,9 @@ class PcgHandle {
 
   private:
     static constexpr uint64_t kMaxRecords = 1ull << 20;  // 64 MiB of records at most
+    // inputs from this many stored entries are sorted into CSR on the GPU
+    // (same CSR as the host build; below it the host build is quicker)
+    static constexpr uint32_t kDeviceIngestMin = 1u << 16;
 
     void require_device() {
         if (!ready_) throw Error(XJPCG_STATUS_INTERNAL_ERROR, init_error_.empty() ? "INTERNAL ERROR: handle not initialised." : init_error_);
@@ -324,6 +216,28 @@ class PcgHandle {
         refresh_values(data, P.src_len);
         op_.reset(new Operator((int32_t)n, P.nnz, d_rowptr_.ptr, d_col_.ptr, d_val_.ptr, false, nullptr,
                                P.rowptr.data(), stream_));
+    }
+
+    // CSR built on the GPU by `build` (into d_rowptr_, d_col_, d_perm_), then
+    // values placed and the operator created as for a host-built pattern
+    template <typename Build>
+    void build_on_device(uint32_t n, int64_t nnz, int64_t src_len, const double* data, Build build) {
+        op_.reset();
+        d_rowptr_.ensure((size_t)n + 1);
+        d_col_.ensure((size_t)nnz);
+        d_val_.ensure((size_t)nnz);
+        d_perm_.ensure((size_t)nnz);
+        build(d_rowptr_.ptr, d_col_.ptr, d_perm_.ptr);
+        std::vector<int32_t> rp((size_t)n + 1);
+        hip_check(hipMemcpyAsync(rp.data(), d_rowptr_.ptr, sizeof(int32_t) * rp.size(), hipMemcpyDeviceToHost, stream_),
+                  "copy row pointers");
+        hip_check(hipStreamSynchronize(stream_), "copy row pointers");
+        pat_n_ = n;
+        pat_nnz_ = nnz;
+        src_len_ = src_len;
+        refresh_values(data, src_len);
+        op_.reset(new Operator((int32_t)n, nnz, d_rowptr_.ptr, d_col_.ptr, d_val_.ptr, false, nullptr, rp.data(),
+                               stream_));
     }
 
     // values reach their CSR slots on the GPU: val[k] = data[perm[k]]

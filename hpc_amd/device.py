@@ -272,3 +272,38 @@ def stream_copy(src: torch.Tensor, dst: torch.Tensor):
     _dev_check(src, torch.float64, "src")
     _dev_check(dst, torch.float64, "dst", src.numel())
     check_dev(lib().xjpcg_stream_copy(ptr(src), ptr(dst), src.numel(), _stream()))
+
+
+def csr_from_coo(n: int, row, col, base: int = 0, device=None):
+    """Device CSR from host COO index arrays (numpy uint32-compatible): (rowptr, col, perm) torch tensors.
+
+    Rows sorted, columns ascending within a row, duplicates in input order; perm[k] is the index of entry k's
+    value in the caller's value array (values on the device: val[perm])."""
+    import numpy as np
+    row = np.ascontiguousarray(row, dtype=np.uint32)
+    col = np.ascontiguousarray(col, dtype=np.uint32)
+    nnz = row.size
+    device = device or torch.device("cuda:0")
+    rp = torch.empty(n + 1, dtype=torch.int32, device=device)
+    c = torch.empty(max(nnz, 1), dtype=torch.int32, device=device)
+    pm = torch.empty(max(nnz, 1), dtype=torch.int64, device=device)
+    check_dev(lib().xjpcg_csr_from_coo(n, nnz, row.ctypes.data, col.ctypes.data, int(base), ptr(rp), ptr(c), ptr(pm),
+                                       _stream()))
+    return rp, c[:nnz], pm[:nnz]
+
+
+def csr_from_cscsym(n: int, row_idx, col_ptr, base: int = 0, device=None):
+    """Device CSR from a host half-stored symmetric CSC (int64 arrays): (rowptr, col, perm) torch tensors."""
+    import numpy as np
+    row_idx = np.ascontiguousarray(row_idx, dtype=np.int64)
+    col_ptr = np.ascontiguousarray(col_ptr, dtype=np.int64)
+    nh = row_idx.size
+    full = max(2 * nh - n, 0)
+    device = device or torch.device("cuda:0")
+    rp = torch.empty(n + 1, dtype=torch.int32, device=device)
+    c = torch.empty(max(full, 1), dtype=torch.int32, device=device)
+    pm = torch.empty(max(full, 1), dtype=torch.int64, device=device)
+    m = ctypes.c_int64()
+    check_dev(lib().xjpcg_csr_from_cscsym(n, nh, row_idx.ctypes.data, col_ptr.ctypes.data, int(base), ptr(rp), ptr(c),
+                                          ptr(pm), ctypes.byref(m), _stream()))
+    return rp, c[:m.value], pm[:m.value]

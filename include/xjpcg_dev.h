@@ -101,6 +101,25 @@ XJPCG_Status_t xjpcg_op_info(const xjpcg_op_t* op, int32_t* n, int64_t* nnz, int
  * NULL) receives the loop the last solve used; path < 0 only queries. */
 XJPCG_Status_t xjpcg_op_set_path(xjpcg_op_t* op, int path, int* h_last);
 
+/* Matrix ingest on the GPU (the reference does this on the host:
+ * matrix_params.hpp:51-134 loadCoo / loadCscSym, gen_signature.hpp:72-112).
+ * From the caller's host COO (row, col with index base 0 or 1) build a device
+ * CSR whose rows are sorted, whose columns ascend within a row and whose
+ * duplicate entries keep their input order (the SpMV sums them, pcg.h:189),
+ * plus d_perm[k] = index of entry k's value in the caller's value array, so
+ * values can be placed on the device.  d_rowptr[n+1], d_col[nnz],
+ * d_perm[nnz].  The indices are checked on the host first with the checks
+ * and messages of xJPCG_cooSolver; synchronises `stream`. */
+XJPCG_Status_t xjpcg_csr_from_coo(uint32_t n, uint32_t nnz, const uint32_t* h_row, const uint32_t* h_col, int base,
+                                  int32_t* d_rowptr, int32_t* d_col, int64_t* d_perm, void* stream);
+/* The same from a half-stored symmetric CSC (xJPCG_cscSymSolver's input,
+ * pcg.h:129-133), expanded to 2*nnz_half - n entries (*h_nnz), which d_col
+ * and d_perm must hold; off-diagonal entries appear twice with the same value
+ * index. */
+XJPCG_Status_t xjpcg_csr_from_cscsym(uint32_t n, uint32_t nnz_half, const int64_t* h_row_idx,
+                                     const int64_t* h_col_ptr, int base, int32_t* d_rowptr, int32_t* d_col,
+                                     int64_t* d_perm, int64_t* h_nnz, void* stream);
+
 /* d_dst = d_src for n doubles (16-byte aligned), with 16-byte non-temporal
  * loads and stores: the stream kernel that gives the achievable HBM rate the
  * SpMV is compared with.  No counterpart in the reference. */

@@ -1,0 +1,155 @@
+"""Matrix ingest on the GPU (COO and half-stored symmetric CSC to CSR + value map) against a numpy stable
+sort, its error reporting, and pcg.h solves large enough to take the device path."""
+import sys
+
+import numpy as np
+import pytest
+
+from hpc_amd import workloads
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(m):
+    print(m, flush=True)
+    sys.stdout.flush()
+
+
+def ref_csr(n, rows, cols):
+    """rows sorted, columns ascending within a row, ties in input order (numpy's lexsort is stable)."""
+    order = np.lexsort((np.arange(rows.size), cols, rows))
+    rp = np.zeros(n + 1, np.int64)
+    np.cumsum(np.bincount(rows, minlength=n), out=rp[1:])
+    return rp, cols[order], order
+
+
+def random_coo(n, m, seed):
+    rng = np.random.default_rng(seed)
+    rows = rng.integers(0, n, m)
+    cols = rng.integers(0, n, m)
+    dup = rng.random(m) < 0.05                     # repeat some entries verbatim later in the input
+    rows[dup] = rows[np.nonzero(dup)[0] // 2]
+    cols[dup] = cols[np.nonzero(dup)[0] // 2]
+    return rows, cols
+
+
+@pytest.mark.parametrize("base", [0, 1])
+def test_csr_from_coo_matches_stable_sort(cuda, base):
+    from hpc_amd import device
+    n, m = 50_000, 400_000
+    rows, cols = random_coo(n, m, 3 + base)
+    rp, c, pm = device.csr_from_coo(n, rows + base, cols + base, base, cuda)
+    rrp, rc, rperm = ref_csr(n, rows, cols)
+    assert np.array_equal(rp.cpu().numpy(), rrp)
+    assert np.array_equal(c.cpu().numpy(), rc)
+    assert np.array_equal(pm.cpu().numpy(), rperm)
+
+
+def test_csr_from_coo_empty_rows_and_edges(cuda):
+    from hpc_amd import device
+    n = 1000
+    rows = np.array([999, 0, 999, 500, 0], np.int64)
+    cols = np.array([999, 3, 0, 500, 3], np.int64)
+    rp, c, pm = device.csr_from_coo(n, rows, cols, 0, cuda)
+    rrp, rc, rperm = ref_csr(n, rows, cols)
+    assert np.array_equal(rp.cpu().numpy(), rrp) and np.array_equal(c.cpu().numpy(), rc)
+    assert np.array_equal(pm.cpu().numpy(), rperm)
+
+
+def test_csr_from_coo_reports_first_bad_entry(cuda):
+    from hpc_amd import device
+    from hpc_amd._lib import XjpcgError
+    n = 100
+    rows = np.arange(300) % n
+    cols = np.arange(300) % n
+    rows[250], cols[120] = n, n + 7
+    with pytest.raises(XjpcgError) as e:
+        device.csr_from_coo(n, rows, cols, 0, cuda)
+    assert e.value.status == 3 and "entry 120." in str(e.value)
+
+
+@pytest.mark.parametrize("base", [0, 1])
+def test_csr_from_cscsym_matches_expansion(cuda, base):
+    """Half-stored CSC of a random symmetric pattern (lower triangle with the diagonal, sorted by column)."""
+    from hpc_amd import device
+    n = 30_000
+    rng = np.random.default_rng(7)
+    i = rng.integers(0, n, 200_000)
+    j = rng.integers(0, n, 200_000)
+    lo, hi = np.maximum(i, j), np.minimum(i, j)
+    key = np.unique(lo * n + hi)
+    r, c = key // n, key % n
+    r = np.concatenate([r, np.arange(n)])
+    c = np.concatenate([c, np.arange(n)])
+    key = np.unique(c * n + r)                     # column-major, unique, every diagonal present
+    c, r = key // n, key % n
+    col_ptr = np.zeros(n + 1, np.int64)
+    np.cumsum(np.bincount(c, minlength=n), out=col_ptr[1:])
+    rp, cc, pm = device.csr_from_cscsym(n, r + base, col_ptr + base, base, cuda)
+    # the loader's emission order: entry k gives (i, j) then (j, i) when off the diagonal
+    off = r != c
+    er = np.empty(r.size + off.sum(), np.int64)
+    ec = np.empty_like(er)
+    es = np.empty_like(er)
+    pos = np.concatenate([[0], np.cumsum(1 + off)[:-1]])
+    er[pos], ec[pos], es[pos] = r, c, np.arange(r.size)
+    er[pos[off] + 1], ec[pos[off] + 1], es[pos[off] + 1] = c[off], r[off], np.nonzero(off)[0]
+    rrp, rc, order = ref_csr(n, er, ec)
+    assert cc.numel() == 2 * r.size - n
+    assert np.array_equal(rp.cpu().numpy(), rrp)
+    assert np.array_equal(cc.cpu().numpy(), rc)
+    assert np.array_equal(pm.cpu().numpy(), es[order])
+
+
+def _shuffled_coo(A, seed):
+    rng = np.random.default_rng(seed)
+    rows = np.repeat(np.arange(A.n), np.diff(A.rowptr))
+    cols, vals = A.col.astype(np.int64), A.val.copy()
+    # split a tenth of the entries into two duplicates that sum to the original value
+    sp = rng.random(vals.size) < 0.1
+    half = vals[sp] * 0.5
+    vals[sp] = half
+    rows = np.concatenate([rows, rows[sp]])
+    cols = np.concatenate([cols, cols[sp]])
+    vals = np.concatenate([vals, half])
+    p = rng.permutation(rows.size)
+    return rows[p], cols[p], vals[p]
+
+
+def test_pcg_coo_solver_device_ingest_matches_oracle(cuda):
+    """xJPCG_cooSolver on a shuffled COO with duplicates, large enough for the GPU ingest."""
+    from hpc_amd.pcg import JPCG, Status
+    s = workloads.c1_laplace2d(200)
+    A = s.A
+    rows, cols, vals = _shuffled_coo(A, 11)
+    assert rows.size >= 1 << 16
+    rp, col, val, _ = oracle.coo_to_csr(A.n, rows, cols, vals)
+    ref = oracle.jpcg(rp, col, val, A.diag, s.b, 5000, 1e-12)
+    with JPCG(0) as h:
+        out = h.coo_solve(A.n, rows, cols, vals, A.diag, s.b, 5000, 1e-12)
+    _say(f"coo device ingest: {out.iterations} iterations (oracle {ref['iter']}), status {out.status}")
+    assert out.status == Status.SUCCESS
+    assert abs(out.iterations - ref["iter"]) <= 1
+    assert np.linalg.norm(out.x - ref["x"]) / np.linalg.norm(ref["x"]) <= 1e-10
+
+
+def test_pcg_cscsym_solver_device_ingest_matches_oracle(cuda):
+    from hpc_amd.pcg import JPCG, Status
+    s = workloads.c1_laplace2d(200)
+    A = s.A
+    rows = np.repeat(np.arange(A.n), np.diff(A.rowptr))
+    low = rows >= A.col                            # lower triangle incl. diagonal, column-major = CSC of it
+    r, c, v = rows[low], A.col[low].astype(np.int64), A.val[low]
+    o = np.lexsort((r, c))
+    r, c, v = r[o], c[o], v[o]
+    col_ptr = np.zeros(A.n + 1, np.int64)
+    np.cumsum(np.bincount(c, minlength=A.n), out=col_ptr[1:])
+    assert r.size >= 1 << 16
+    ref = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, s.b, 5000, 1e-12)
+    with JPCG(0) as h:
+        out = h.csc_sym_solve(A.n, r + 1, col_ptr + 1, v, A.diag, s.b, 5000, 1e-12, mode=0x10)
+    _say(f"cscsym device ingest: {out.iterations} iterations (oracle {ref['iter']}), status {out.status}")
+    assert out.status == Status.SUCCESS
+    assert abs(out.iterations - ref["iter"]) <= 1
+    assert np.linalg.norm(out.x - ref["x"]) / np.linalg.norm(ref["x"]) <= 1e-10
