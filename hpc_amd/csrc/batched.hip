@@ -118,43 +118,120 @@ __global__ __launch_bounds__(kMaxBlock) void bsys_setup(BatchParams P) {
     }
 }
 
+// SpMM lane mapping: a lane owns one row and Q = min(R, 4) consecutive
+// right-hand sides (L = R / Q lanes per row), so a neighbour's values are one
+// 8Q-byte vector load, the column index and value of an entry are loaded by L
+// lanes instead of R, and the index arithmetic is done once per Q outputs.
+// Entries are staged kRowStage at a time (indices and values first, then the
+// gathers, then the adds), so a lane's gathers are in flight together.
+// Each (row, rhs) is summed left to right from dsplit[i]*x (or 0).
+constexpr int kRowStage = 8;
+
+template <int R>
+struct SpmmLanes {
+    static constexpr int Q = R < 4 ? R : 4;
+    static constexpr int L = R / Q;
+};
+
+template <int Q>
+__device__ __forceinline__ void load_q(const double* __restrict__ p, double (&out)[Q]) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) out[q] = p[q];
+}
+
+// acc[q] = row i of A times column (q0 + q) of the [N][R] block xin
+template <typename VT, int R>
+__device__ __forceinline__ void brow(const BatchParams& P, const VT* __restrict__ val, const double* __restrict__ xin,
+                                     int i, int q0, double (&acc)[SpmmLanes<R>::Q]) {
+    constexpr int Q = SpmmLanes<R>::Q;
+    const int k0 = P.rowptr[i], k1 = P.rowptr[i + 1];
+    if (P.dsplit) {
+        double xi[Q];
+        load_q<Q>(xin + (size_t)i * R + q0, xi);
+        const double d = P.dsplit[i];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) acc[q] = d * xi[q];
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) acc[q] = 0.0;
+    }
+    for (int kb = k0; kb < k1; kb += kRowStage) {
+        int c[kRowStage];
+        double v[kRowStage];
+#pragma unroll
+        for (int u = 0; u < kRowStage; ++u) {
+            const int k = kb + u < k1 ? kb + u : k0;
+            c[u] = ld_stream(P.col + k);
+            v[u] = (double)ld_stream(val + k);
+        }
+        double g[kRowStage][Q];
+#pragma unroll
+        for (int u = 0; u < kRowStage; ++u) load_q<Q>(xin + (size_t)c[u] * R + q0, g[u]);
+#pragma unroll
+        for (int u = 0; u < kRowStage; ++u)
+            if (kb + u < k1) {
+#pragma unroll
+                for (int q = 0; q < Q; ++q) acc[q] += v[u] * g[u][q];
+            }
+    }
+}
+
 template <typename VT, int R>
 __global__ __launch_bounds__(kMaxBlock) void bspmm_dot(BatchParams P) {
-    __shared__ double red[kMaxBlock];
+    constexpr int Q = SpmmLanes<R>::Q, L = SpmmLanes<R>::L;
+    __shared__ double red[kMaxBlock * Q];
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int s = P.tile_sys[t];
     if (!P.sys_active[s]) return;
-    const int j = threadIdx.x % R, g = threadIdx.x / R, G = blockDim.x / R;
+    const int l = threadIdx.x % L, g = threadIdx.x / L, G = blockDim.x / L;
+    const int q0 = l * Q;
     const int r0 = P.tile_row[t], r1 = P.tile_row[t + 1];
     const VT* __restrict__ val = static_cast<const VT*>(P.val);
     const double* __restrict__ p = P.p;
-    double dl = 0.0;
+    double dl[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) dl[q] = 0.0;
     for (int i = r0 + g; i < r1; i += G) {
-        const int k0 = P.rowptr[i], k1 = P.rowptr[i + 1];
-        double acc = P.dsplit ? P.dsplit[i] * p[(size_t)i * R + j] : 0.0;
-        for (int k = k0; k < k1; ++k) acc += (double)ld_stream(val + k) * p[(size_t)ld_stream(P.col + k) * R + j];
-        P.Ap[(size_t)i * R + j] = acc;
-        dl += p[(size_t)i * R + j] * acc;
+        double acc[Q];
+        brow<VT, R>(P, val, p, i, q0, acc);
+        double pi[Q];
+        load_q<Q>(p + (size_t)i * R + q0, pi);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            P.Ap[(size_t)i * R + q0 + q] = acc[q];
+            dl[q] += pi[q] * acc[q];
+        }
     }
-    dl = group_sum<R>(dl, red);
-    if (g == 0) P.partA[(size_t)t * R + j] = dl;
+    // per right-hand side: sum over the row groups in fixed order g = 0, 1, ...
+#pragma unroll
+    for (int q = 0; q < Q; ++q) red[g * R + q0 + q] = dl[q];
+    __syncthreads();
+    if ((int)threadIdx.x < R) {
+        const int j = threadIdx.x;
+        double sum = 0.0;
+        for (int gg = 0; gg < G; ++gg) sum += red[gg * R + j];
+        P.partA[(size_t)t * R + j] = sum;
+    }
 }
 
 // plain batched SpMM y = A x (or y = b - A x) for the autotuner and the refinement residual
 template <typename VT, int R>
 __global__ __launch_bounds__(kMaxBlock) void bspmm(BatchParams P, const double* __restrict__ xin,
                                                    double* __restrict__ y, const double* __restrict__ b) {
+    constexpr int Q = SpmmLanes<R>::Q, L = SpmmLanes<R>::L;
     const int t = xcd_remap(blockIdx.x, gridDim.x);
-    const int j = threadIdx.x % R, g = threadIdx.x / R, G = blockDim.x / R;
+    const int l = threadIdx.x % L, g = threadIdx.x / L, G = blockDim.x / L;
+    const int q0 = l * Q;
     const int r0 = P.tile_row[t], r1 = P.tile_row[t + 1];
     const VT* __restrict__ val = static_cast<const VT*>(P.val);
     for (int i = r0 + g; i < r1; i += G) {
-        const int k0 = P.rowptr[i], k1 = P.rowptr[i + 1];
-        double acc = P.dsplit ? P.dsplit[i] * xin[(size_t)i * R + j] : 0.0;
-        for (int k = k0; k < k1; ++k)
-            acc += (double)ld_stream(val + k) * xin[(size_t)ld_stream(P.col + k) * R + j];
-        const size_t e = (size_t)i * R + j;
-        y[e] = b ? b[e] - acc : acc;
+        double acc[Q];
+        brow<VT, R>(P, val, xin, i, q0, acc);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const size_t e = (size_t)i * R + q0 + q;
+            y[e] = b ? b[e] - acc[q] : acc[q];
+        }
     }
 }
 
