@@ -555,6 +555,92 @@ __global__ __launch_bounds__(kBlock) void spmv_pattern(int ntiles, int nnz, cons
     }
 }
 
+// Column-offset pattern plan on the GPU, one workgroup per tile (the same
+// plan as the host planner plan_patterns): lane t checks row t (at most 32
+// entries, columns strictly ascending and inside [0, n)) and inserts its
+// offsets col - row into a 64-slot LDS hash set; the tile's distinct offsets
+// (at most 32) are ranked into ascending order, written to poffs[t*32 ..]
+// (unused slots 0), and lane t builds row t's mask against them.  Any tile
+// that cannot be held sets *fail.
+__global__ __launch_bounds__(kBlock) void plan_pattern_tiles(int n, int ntiles, const int* __restrict__ tile_row,
+                                                             const int* __restrict__ rowptr,
+                                                             const int* __restrict__ col, uint32_t* __restrict__ pmask,
+                                                             int* __restrict__ poffs, int* __restrict__ fail) {
+    constexpr int kSlots = 64;
+    constexpr int kEmpty = INT32_MIN;  // no offset col - row reaches it for n < 2^31
+    __shared__ int tab[kSlots];
+    __shared__ int sorted[32];
+    __shared__ int cnt, bad;
+    const int t = blockIdx.x;
+    if (t >= ntiles) return;
+    const int tid = threadIdx.x;
+    const int r0 = tile_row[t], r1 = tile_row[t + 1];
+    if (tid < kSlots) tab[tid] = kEmpty;
+    if (tid == 0) {
+        cnt = 0;
+        bad = rowptr[r1] - rowptr[r0] > kTilePlanNnz ? 1 : 0;
+    }
+    __syncthreads();
+    const bool mine = tid < r1 - r0;
+    const int i = r0 + tid;
+    const int a = mine ? rowptr[i] : 0, b = mine ? rowptr[i + 1] : 0;
+    if (mine && !bad) {
+        if (b - a > 32) atomicOr(&bad, 1);
+        int prev = -1;
+        for (int k = a; k < b && !bad; ++k) {
+            const int c = col[k];
+            if (c < 0 || c >= n || c <= prev) {
+                atomicOr(&bad, 1);
+                break;
+            }
+            prev = c;
+            const int d = c - i;
+            int h = (int)(((uint32_t)d * 2654435761u) >> 26);  // 6-bit multiplicative hash
+            for (int probe = 0; probe < kSlots; ++probe) {
+                const int old = atomicCAS(&tab[h], kEmpty, d);
+                if (old == kEmpty) {
+                    if (atomicAdd(&cnt, 1) >= 32) atomicOr(&bad, 1);
+                    break;
+                }
+                if (old == d) break;
+                h = (h + 1) & (kSlots - 1);
+            }
+        }
+    }
+    __syncthreads();
+    if (bad) {
+        if (tid == 0) atomicOr(fail, 1);
+        return;
+    }
+    if (tid < kSlots) {
+        const int v = tab[tid];
+        if (v != kEmpty) {
+            int rank = 0;
+            for (int u = 0; u < kSlots; ++u) {
+                const int w = tab[u];
+                rank += (w != kEmpty && w < v) ? 1 : 0;
+            }
+            sorted[rank] = v;
+        }
+    }
+    __syncthreads();
+    const int m = cnt;
+    if (tid < 32) poffs[t * 32 + tid] = tid < m ? sorted[tid] : 0;
+    if (mine) {
+        uint32_t mask = 0;
+        for (int k = a; k < b; ++k) {
+            const int d = col[k] - i;
+            int lo = 0, hi = m - 1;
+            while (lo < hi) {  // sorted[lo..hi] holds d
+                const int mid = (lo + hi) >> 1;
+                if (sorted[mid] < d) lo = mid + 1; else hi = mid;
+            }
+            mask |= 1u << lo;
+        }
+        pmask[i] = mask;
+    }
+}
+
 // col16[k] = col[k] - first row of k's tile; *overflow = 1 if any difference leaves int16
 __global__ __launch_bounds__(kBlock) void build_col16(int ntiles, const int* __restrict__ tile_row,
                                                       const int* __restrict__ rowptr, const int* __restrict__ col,
@@ -750,6 +836,12 @@ hipError_t launch_spmv(const CsrDev& A, const double* x, double* y, const double
                                                                    static_cast<const CT*>(col),
                                                                    static_cast<const VT*>(A.val), A.dsplit, x, y,
                                                                    b)));
+    return hipGetLastError();
+}
+
+hipError_t launch_plan_patterns(const CsrDev& A, uint32_t* pmask, int* poffs, int* fail, hipStream_t s) {
+    if (A.ntiles <= 0) return hipSuccess;
+    plan_pattern_tiles<<<A.ntiles, kBlock, 0, s>>>(A.n, A.ntiles, A.tile_row, A.rowptr, A.col, pmask, poffs, fail);
     return hipGetLastError();
 }
 

@@ -126,6 +126,7 @@ hipError_t persistent_grid(bool f32, bool c16, int* G);
 hipError_t launch_jpcg(const JpcgParams& P, bool f32, bool c16, int G, hipStream_t s);
 hipError_t launch_spmv(const CsrDev& A, const double* x, double* y, const double* b, hipStream_t s);
 hipError_t launch_build_col16(const CsrDev& A, int16_t* col16, int* overflow, hipStream_t s);
+hipError_t launch_plan_patterns(const CsrDev& A, uint32_t* pmask, int* poffs, int* fail, hipStream_t s);
 int dot_blocks(int n);
 hipError_t launch_dot(int n, const double* a, const double* b, double* part, double* out, hipStream_t s);
 hipError_t launch_gather_f64(int64_t nnz, const int64_t* perm, const double* src, double* dst, hipStream_t s);

@@ -226,18 +226,16 @@ Operator::Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t*
     // XCD's L2, the panel layout.  The faster layout stays (panels only when
     // at least 15% faster than the tiles).
     if (nnz > 0) {
-        std::vector<int32_t> rp_h, col_h;
-        copy_pattern_to_host(s, rp_h, col_h);
         const float t_tiles = time_layout(kTiles, s);
         float best = t_tiles;
-        if (build_patterns(s, rp_h.data(), col_h.data())) {
+        if (build_patterns(s)) {
             const float t_pat = time_layout(kPatterns, s);
             if (t_pat > 0.0f && t_pat < best) {
                 best = t_pat;
                 auto_layout_ = kPatterns;
             }
         }
-        if (!has_col16_ && n >= kPanelAutoRows && build_panels(s, rp_h.data(), col_h.data())) {
+        if (!has_col16_ && n >= kPanelAutoRows && build_panels(s)) {
             const float t_pan = time_layout(kPanels, s);
             if (t_pan > 0.0f && t_pan < 0.85f * t_tiles && t_pan < best) {
                 best = t_pan;
@@ -265,30 +263,30 @@ void Operator::copy_pattern_to_host(hipStream_t s, std::vector<int32_t>& rp, std
     hip_check(hipStreamSynchronize(s), "copy pattern");
 }
 
-bool Operator::build_patterns(hipStream_t s, const int32_t* rp_host, const int32_t* col_host) {
+// plan on the GPU (plan_pattern_tiles); no copy of the pattern to the host
+bool Operator::build_patterns(hipStream_t s) {
     if (has_pat_) return true;
     if (pat_tried_) return false;
     pat_tried_ = true;
-    std::vector<int32_t> rp_l, col_l;
-    if (!rp_host || !col_host) copy_pattern_to_host(s, rp_l, col_l);
-    const int32_t* rp = rp_host && col_host ? rp_host : rp_l.data();
-    const int32_t* col = rp_host && col_host ? col_host : col_l.data();
-    std::vector<uint32_t> mask;
-    std::vector<int32_t> offs;
-    if (!plan_patterns(A_.n, rp, col, tile_plan_, mask, offs)) return false;
     try {
-        pmask_.ensure(mask.size());
-        poffs_.ensure(offs.size());
+        pmask_.ensure((size_t)A_.n);
+        poffs_.ensure((size_t)std::max(1, A_.ntiles) * 32);
     } catch (const Error& e) {
         if (e.status != XJPCG_STATUS_ALLOC_FAILED) throw;
         free_patterns();
+        pat_tried_ = true;
         return false;
     }
-    hip_check(hipMemcpyAsync(pmask_.ptr, mask.data(), sizeof(uint32_t) * mask.size(), hipMemcpyHostToDevice, s),
-              "upload row masks");
-    hip_check(hipMemcpyAsync(poffs_.ptr, offs.data(), sizeof(int32_t) * offs.size(), hipMemcpyHostToDevice, s),
-              "upload tile offsets");
-    hip_check(hipStreamSynchronize(s), "upload pattern tiles");
+    hip_check(hipMemsetAsync(abort_.ptr, 0, sizeof(int), s), "reset flag");
+    hip_check(launch_plan_patterns(A_, pmask_.ptr, poffs_.ptr, abort_.ptr, s), "pattern plan");
+    int fail = 1;
+    hip_check(hipMemcpyAsync(&fail, abort_.ptr, sizeof(int), hipMemcpyDeviceToHost, s), "pattern plan");
+    hip_check(hipStreamSynchronize(s), "pattern plan");
+    if (fail) {
+        free_patterns();
+        pat_tried_ = true;
+        return false;
+    }
     A_.pmask = pmask_.ptr;
     A_.poffs = poffs_.ptr;
     has_pat_ = true;

@@ -163,7 +163,7 @@ class Operator {
     void drop_graph();
     // panel layout: build (host plan + upload), release, refresh values, time
     bool build_panels(hipStream_t s, const int32_t* rp_host = nullptr, const int32_t* col_host = nullptr);
-    bool build_patterns(hipStream_t s, const int32_t* rp_host = nullptr, const int32_t* col_host = nullptr);
+    bool build_patterns(hipStream_t s);
     void copy_pattern_to_host(hipStream_t s, std::vector<int32_t>& rp, std::vector<int32_t>& col) const;
     float time_layout(int layout, hipStream_t s);
     void free_panels();

@@ -180,3 +180,51 @@ def test_stream_copy(cuda):
         dst = torch.full_like(src, -1.0)
         device.stream_copy(src, dst)
         assert bool((dst == src).all())
+
+
+def _band_csr(n, offsets_even, offsets_odd, seed=9):
+    rng = np.random.default_rng(seed)
+    rows, cols = [], []
+    for i in range(n):
+        offs = offsets_even if i % 2 == 0 else offsets_odd
+        c = i + np.asarray(offs)
+        c = c[(c >= 0) & (c < n)]
+        rows.append(np.full(c.size, i))
+        cols.append(np.sort(c))
+    rows, cols = np.concatenate(rows), np.concatenate(cols)
+    rp = np.zeros(n + 1, np.int32)
+    np.cumsum(np.bincount(rows, minlength=n), out=rp[1:])
+    return rp, cols.astype(np.int32), rng.standard_normal(cols.size)
+
+
+@pytest.mark.parametrize("case", ["3d7-aniso", "diffusion", "band32"])
+def test_pattern_tiles_more_operators(cuda, case):
+    """Pattern plans built on the GPU: SpMV bit-exact against the oracle on more stencil shapes, including a
+    band of exactly 32 offsets."""
+    from hpc_amd import device
+    if case == "3d7-aniso":
+        A = workloads.stencil(workloads.STENCIL_3D7, 33, 17, 9, [0.3, 1.7, 2.9])
+        rp, col, val = A.rowptr, A.col, A.val
+    elif case == "diffusion":
+        A, _, _ = workloads.c5_diffusion(m=64, length=4.0, seed=2)
+        rp, col, val = A.rowptr, A.col, A.val
+    else:
+        rp, col, val = _band_csr(6000, list(range(-16, 16)), list(range(-16, 16)))
+    n = rp.size - 1
+    op = device.CsrOperator(_t(cuda, rp), _t(cuda, col), _t(cuda, val), rowptr_host=rp)
+    assert op.set_layout(3) == 3
+    xv = workloads.uniform(8, 1, n, -1, 1)
+    assert np.array_equal(op.spmv(_t(cuda, xv)).cpu().numpy(), oracle.spmv(rp, col, val, xv))
+
+
+def test_pattern_layout_refused_beyond_32_offsets(cuda):
+    """Rows of 32 entries each, but 48 distinct offsets per tile: the GPU plan must refuse, as the host plan."""
+    from hpc_amd import device
+    from hpc_amd._lib import XjpcgError
+    rp, col, val = _band_csr(4000, list(range(16)) + list(range(100, 116)), list(range(16)) + list(range(200, 216)))
+    assert device.pattern_plan(rp, col) is None
+    op = device.CsrOperator(_t(cuda, rp), _t(cuda, col), _t(cuda, val), rowptr_host=rp)
+    with pytest.raises(XjpcgError):
+        op.set_layout(3)
+    xv = workloads.uniform(8, 2, rp.size - 1, -1, 1)
+    assert np.array_equal(op.spmv(_t(cuda, xv)).cpu().numpy(), oracle.spmv(rp, col, val, xv))
