@@ -31,6 +31,12 @@ _SIGNATURES = {
     "xJPCG_getMetrics": (c_int, [c_void_p, c_void_p]),
     # xjpcg_ext.h
     "xJPCG_getHistory": (c_int, [c_void_p, c_void_p, c_uint64, POINTER(c_uint64)]),
+    "xJPCG_setCooMat": (c_int, [c_void_p, c_uint32, c_uint32, c_void_p, c_void_p, c_void_p, c_int]),
+    "xJPCG_setCscSymMat": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int]),
+    "xJPCG_setVec": (c_int, [c_void_p, c_uint32, c_void_p, c_void_p]),
+    "xJPCG_run": (c_int, [c_void_p, c_uint32, c_double, POINTER(c_uint32), POINTER(c_double)]),
+    "xJPCG_getResults": (c_int, [c_void_p, POINTER(c_uint32), POINTER(c_double)]),
+    "xJPCG_getX": (c_int, [c_void_p, c_void_p, c_uint32]),
     # xjpcg_dev.h
     "xjpcg_dev_last_error": (c_char_p, []),
     "xjpcg_op_create": (c_int, [POINTER(c_void_p), c_int32, c_int64, c_void_p, c_void_p, c_void_p, c_int, c_void_p,

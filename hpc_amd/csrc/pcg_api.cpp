@@ -153,9 +153,23 @@ class PcgHandle {
         hip_check(hipMemcpyAsync(d_diag_.ptr, diag, sizeof(double) * n, hipMemcpyHostToDevice, stream_),
                   "upload diagonal");
         hip_check(hipStreamSynchronize(stream_), "upload vectors");
+        vec_n_ = n;
     }
 
-    // reference PCGImpl::run (pcgImp.hpp:157-183): solve, then copy x out
+    // results of the last solve (Results, pcgImp.hpp:30-35)
+    bool has_result() const { return has_result_; }
+    const SolveResult& last_result() const { return last_result_; }
+    void copy_x(double* out, uint32_t n) {
+        if (!has_result_) throw invalid("no solve has run on this handle.");
+        if (n != pat_n_) throw invalid("vector size does not match the matrix.");
+        require_device();
+        hip_check(hipMemcpyAsync(out, d_x_.ptr, sizeof(double) * pat_n_, hipMemcpyDeviceToHost, stream_), "copy x");
+        hip_check(hipStreamSynchronize(stream_), "copy x");
+    }
+    bool has_vectors() const { return vec_n_ != 0 && vec_n_ == pat_n_ && op_ != nullptr; }
+
+    // reference PCGImpl::run (pcgImp.hpp:157-183): solve, then copy x out (x_out may be null: x stays on
+    // the GPU)
     SolveResult run(uint32_t max_iter, double tol, double* x_out) {
         require_device();
         void* rec = nullptr;
@@ -165,12 +179,16 @@ class PcgHandle {
             rec = d_records_.ptr;
         }
         SolveResult R = op_->solve(d_diag_.ptr, d_b_.ptr, d_x_.ptr, max_iter, tol, rec, stream_);
-        hip_check(hipMemcpyAsync(x_out, d_x_.ptr, sizeof(double) * pat_n_, hipMemcpyDeviceToHost, stream_),
-                  "copy x");
-        hip_check(hipStreamSynchronize(stream_), "copy x");
+        if (x_out) {
+            hip_check(hipMemcpyAsync(x_out, d_x_.ptr, sizeof(double) * pat_n_, hipMemcpyDeviceToHost, stream_),
+                      "copy x");
+            hip_check(hipStreamSynchronize(stream_), "copy x");
+        }
         records_valid_ = rec != nullptr;
         last_executed_ = R.executed;
         first_call_ = false;
+        last_result_ = R;
+        has_result_ = true;
         return R;
     }
 
@@ -268,6 +286,9 @@ class PcgHandle {
     DevBuf<uint8_t> d_records_;
     bool records_valid_ = false;
     int64_t last_executed_ = 0;
+    uint32_t vec_n_ = 0;
+    bool has_result_ = false;
+    SolveResult last_result_;
     std::unique_ptr<Operator> op_;
 };
 
@@ -386,6 +407,83 @@ XJPCG_Status_t xJPCG_cooSolver(XJPCG_Handle_t* handle, const uint32_t p_n, const
         if (*p_res > p_tol)
             throw xj::exec_failed("exit with divergent solution after " + std::to_string(*p_iter) + " iterations.");
     }, "Solver returns successfully");
+}
+
+// The solve steps as separate calls (PCGImpl, pcgImp.hpp:49-204).
+XJPCG_Status_t xJPCG_setCooMat(XJPCG_Handle_t* handle, uint32_t n, uint32_t nnz, const uint32_t* rowIdx,
+                               const uint32_t* colIdx, const double* data, XJPCG_Mode_t mode) {
+    if (handle == nullptr) return XJPCG_STATUS_NOT_INITIALIZED;
+    PcgHandle* h = as_impl(handle);
+    return run_guarded(h, [&] {
+        auto last = xj::Clock::now();
+        switch (mode & 0x0f) {
+            case XJPCG_MODE_DEFAULT: h->set_coo(n, nnz, rowIdx, colIdx, data, (mode & 0xf0) >> 4); break;
+            case XJPCG_MODE_KEEP_NZ_LAYOUT: h->update_values(n, nnz, data); break;
+            case XJPCG_MODE_KEEP_MATRIX: break;
+            default: throw xj::invalid("unknown solver mode.");
+        }
+        h->metrics()->m_matProc = xj::elapsed(last);
+    }, "Matrix is loaded successfully");
+}
+
+XJPCG_Status_t xJPCG_setCscSymMat(XJPCG_Handle_t* handle, int64_t n, int64_t nnz, const int64_t* rowIdx,
+                                  const int64_t* colPtr, const double* data, XJPCG_Mode_t mode) {
+    if (handle == nullptr) return XJPCG_STATUS_NOT_INITIALIZED;
+    PcgHandle* h = as_impl(handle);
+    return run_guarded(h, [&] {
+        auto last = xj::Clock::now();
+        const uint32_t n32 = (uint32_t)n, nnz32 = (uint32_t)nnz;  // PCGImpl takes uint32 sizes
+        switch (mode & 0x0f) {
+            case XJPCG_MODE_DEFAULT: h->set_cscsym(n32, nnz32, rowIdx, colPtr, data, (mode & 0xf0) >> 4); break;
+            case XJPCG_MODE_KEEP_NZ_LAYOUT: h->update_cscsym(n32, nnz32, data); break;
+            case XJPCG_MODE_KEEP_MATRIX: break;
+            default: throw xj::invalid("unknown solver mode.");
+        }
+        h->metrics()->m_matProc = xj::elapsed(last);
+    }, "Matrix is loaded successfully");
+}
+
+XJPCG_Status_t xJPCG_setVec(XJPCG_Handle_t* handle, uint32_t n, const double* b, const double* diagA) {
+    if (handle == nullptr) return XJPCG_STATUS_NOT_INITIALIZED;
+    PcgHandle* h = as_impl(handle);
+    return run_guarded(h, [&] {
+        auto last = xj::Clock::now();
+        h->set_vec(n, b, diagA);
+        h->metrics()->m_vecProc = xj::elapsed(last);
+    }, "Vectors are loaded successfully");
+}
+
+XJPCG_Status_t xJPCG_run(XJPCG_Handle_t* handle, uint32_t maxIter, double tol, uint32_t* iter, double* res) {
+    if (handle == nullptr) return XJPCG_STATUS_NOT_INITIALIZED;
+    PcgHandle* h = as_impl(handle);
+    return run_guarded(h, [&] {
+        if (!h->has_vectors()) throw xj::invalid("load the matrix and the vectors before running the solver.");
+        auto last = xj::Clock::now();
+        const xj::SolveResult R = h->run(maxIter, tol, nullptr);
+        if (iter) *iter = R.iter;
+        if (res) *res = R.res;
+        h->metrics()->m_solver = xj::elapsed(last);
+        if (R.res > tol)
+            throw xj::exec_failed("exit with divergent solution after " + std::to_string(R.iter) + " iterations.");
+    }, "Solver returns successfully");
+}
+
+XJPCG_Status_t xJPCG_getResults(const XJPCG_Handle_t* handle, uint32_t* iter, double* res) {
+    if (handle == nullptr) return XJPCG_STATUS_NOT_INITIALIZED;
+    const PcgHandle* h = as_impl(handle);
+    if (!h->has_result()) return h->set_status(XJPCG_STATUS_INVALID_VALUE, "INVALID VALUE ERROR: no solve has run on this handle.");
+    if (iter) *iter = h->last_result().iter;
+    if (res) *res = h->last_result().res;
+    return h->set_status(XJPCG_STATUS_SUCCESS, "Get results successfully");
+}
+
+XJPCG_Status_t xJPCG_getX(const XJPCG_Handle_t* handle, double* x, uint32_t n) {
+    if (handle == nullptr) return XJPCG_STATUS_NOT_INITIALIZED;
+    PcgHandle* h = const_cast<PcgHandle*>(as_impl(handle));
+    return run_guarded(h, [&] {
+        if (x == nullptr) throw xj::invalid("output pointer is nullptr.");
+        h->copy_x(x, n);
+    }, "Get x successfully");
 }
 
 // reference pcg.cpp:175-180

@@ -162,3 +162,45 @@ class JPCG:
                                         ptr(diag), ptr(b), ptr(x), int(max_iter), float(tol), ctypes.byref(it),
                                         ctypes.byref(res), int(mode))
         return SolveOutput(Status(st), x, it.value, res.value)
+
+    # -- the solve steps as separate calls (xjpcg_ext.h; the reference's PCGImpl methods) -----------------
+    def set_coo_mat(self, n, row, col, data, mode=Mode.DEFAULT) -> Status:
+        """xJPCG_setCooMat: load a COO matrix (DEFAULT) or refresh its values (KEEP_NZ_LAYOUT)."""
+        row = _arr(row, np.uint32)
+        col = _arr(col, np.uint32)
+        data = _arr(data, np.float64)
+        return Status(self._L.xJPCG_setCooMat(self._h, int(n), 0 if data is None else int(data.size), ptr(row),
+                                              ptr(col), ptr(data), int(mode)))
+
+    def set_csc_sym_mat(self, n, row_idx, col_ptr, data, mode=Mode.DEFAULT) -> Status:
+        """xJPCG_setCscSymMat: load a half-stored symmetric CSC matrix or refresh its values."""
+        row_idx = _arr(row_idx, np.int64)
+        col_ptr = _arr(col_ptr, np.int64)
+        data = _arr(data, np.float64)
+        return Status(self._L.xJPCG_setCscSymMat(self._h, int(n), 0 if data is None else int(data.size),
+                                                 ptr(row_idx), ptr(col_ptr), ptr(data), int(mode)))
+
+    def set_vec(self, n, b, diag) -> Status:
+        """xJPCG_setVec: right-hand side and Jacobi diagonal."""
+        b = _arr(b, np.float64)
+        diag = _arr(diag, np.float64)
+        return Status(self._L.xJPCG_setVec(self._h, int(n), ptr(b), ptr(diag)))
+
+    def run(self, max_iter, tol):
+        """xJPCG_run: solve from x0 = 0; returns (status, iterations, residual)."""
+        it = ctypes.c_uint32(0)
+        res = ctypes.c_double(0.0)
+        st = self._L.xJPCG_run(self._h, int(max_iter), float(tol), ctypes.byref(it), ctypes.byref(res))
+        return Status(st), it.value, res.value
+
+    def results(self):
+        """xJPCG_getResults: (status, iterations, residual) of the last solve."""
+        it = ctypes.c_uint32(0)
+        res = ctypes.c_double(0.0)
+        st = self._L.xJPCG_getResults(self._h, ctypes.byref(it), ctypes.byref(res))
+        return Status(st), it.value, res.value
+
+    def get_x(self, n):
+        """xJPCG_getX: (status, x) of the last solve."""
+        x = np.zeros(int(n), dtype=np.float64)
+        return Status(self._L.xJPCG_getX(self._h, x.ctypes.data, int(n))), x

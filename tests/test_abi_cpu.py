@@ -166,3 +166,56 @@ def test_dev_api_validation_without_gpu():
     assert L.xjpcg_op_destroy(None) == Status.NOT_INITIALIZED
     assert L.xjpcg_spmv(None, None, None, None) == Status.INVALID_VALUE  # null vectors checked first
     assert L.xjpcg_solve(None, None, None, None, 1, 1e-8, None, None, None, None, None) == Status.NOT_INITIALIZED
+
+
+def test_reference_library_names():
+    """libXilinxPcg.so / libXilinxPcgStatic.so (the reference's names) are this library."""
+    for name in ("libXilinxPcg.so", "libXilinxPcgStatic.so"):
+        path = os.path.join(ROOT, "hpc_amd", name)
+        assert os.path.islink(path) and os.path.realpath(path) == os.path.realpath(_lib.LIB_PATH)
+        L = ctypes.CDLL(path)
+        assert hasattr(L, "xJPCG_cooSolver") and hasattr(L, "xJPCG_getErrorString")
+
+
+REF_EXAMPLES = [os.path.join(ROOT, "oracle", "_ref", n) for n in ("ref_example_linked", "ref_example_loader")]
+
+
+@pytest.mark.parametrize("exe", REF_EXAMPLES, ids=["linked", "loader"])
+def test_reference_example_binaries_load_library_without_gpu(exe):
+    """The reference's own examples/c/example.c, built against the reference pcg.h and linked with
+    -lXilinxPcg or the reference's dlopen loader, reach this library; without a GPU, createHandle reports why."""
+    if not os.path.exists(exe):
+        pytest.skip("reference example not built (reference tree absent)")
+    if has_gpu():
+        pytest.skip("checks the no-GPU message")
+    env = dict(os.environ, LD_LIBRARY_PATH=os.path.join(ROOT, "hpc_amd"))
+    p = subprocess.run([exe, "0"], env=env, capture_output=True, text=True, timeout=60)
+    assert p.returncode == Status.INTERNAL_ERROR, p.stdout + p.stderr
+    assert "INTERNAL ERROR: no HIP device available." in p.stderr
+
+
+def test_step_api_errors():
+    """xJPCG_setCooMat / setVec / run / getResults / getX: null handles, argument checks and call order."""
+    L = _lib.lib()
+    it, res = ctypes.c_uint32(), ctypes.c_double()
+    assert L.xJPCG_setCooMat(None, 4, 4, None, None, None, 0) == Status.NOT_INITIALIZED
+    assert L.xJPCG_setCscSymMat(None, 4, 4, None, None, None, 0) == Status.NOT_INITIALIZED
+    assert L.xJPCG_setVec(None, 4, None, None) == Status.NOT_INITIALIZED
+    assert L.xJPCG_run(None, 10, 1e-8, ctypes.byref(it), ctypes.byref(res)) == Status.NOT_INITIALIZED
+    assert L.xJPCG_getResults(None, ctypes.byref(it), ctypes.byref(res)) == Status.NOT_INITIALIZED
+    assert L.xJPCG_getX(None, None, 4) == Status.NOT_INITIALIZED
+    n, row, col, val, diag, b = small_coo()
+    with JPCG() as h:
+        assert h.set_coo_mat(0, row, col, val) == Status.INVALID_VALUE
+        assert h.last_message == "INVALID VALUE ERROR: Wrong dimension size."
+        assert h.set_coo_mat(n, row, col, val, 0x03) == Status.INVALID_VALUE
+        assert h.last_message == "INVALID VALUE ERROR: unknown solver mode."
+        st, _, _ = h.run(10, 1e-8)
+        assert st == Status.INVALID_VALUE and "before running the solver" in h.last_message
+        st, _, _ = h.results()
+        assert st == Status.INVALID_VALUE and "no solve has run" in h.last_message
+        st, _ = h.get_x(n)
+        assert st == Status.INVALID_VALUE and "no solve has run" in h.last_message
+        if not has_gpu():
+            assert h.set_coo_mat(n, row, col, val) == Status.INTERNAL_ERROR
+            assert "no HIP device available" in h.last_message
