@@ -350,6 +350,82 @@ def bench_ingest(args):
                       **out, "oracle_coo_to_csr_s": round(cpu, 3)}), flush=True)
 
 
+def bench_signature(args):
+    """Signature files (the reference's partitioned-matrix format) of the 27-point operator (--grid): encode on
+    the host, store / load, then to a device CSR with the streams decoded on the GPU, beside the host decode;
+    the CSR is checked entry for entry against the generator's.  The reference's own partitioner (oracle/_ref,
+    where built) and this encoder are timed on a bounded 40^3 sample."""
+    import shutil
+    import tempfile
+
+    import torch
+    from hpc_amd import workloads
+    from hpc_amd.signature import Signature
+    from oracle import oracle
+    A = workloads.c2_stencil27(args.grid).A
+    rows = np.repeat(np.arange(A.n, dtype=np.uint32), np.diff(A.rowptr))
+    cols = A.col.astype(np.uint32)
+    out = {}
+    t0 = time.perf_counter()
+    sig = Signature.from_coo(A.n, A.n, rows, cols, A.val)
+    out["encode_s"] = round(time.perf_counter() - t0, 3)
+    info = sig.info()
+    sbytes = sum(len(sig.buffer(c)) for c in range(16))
+    log(f"encoded n={A.n} nnz={A.nnz}: {info}, streams {sbytes / 1e9:.2f} GB in {out['encode_s']} s")
+    root = "/dev/shm" if os.path.isdir("/dev/shm") else None
+    d = tempfile.mkdtemp(prefix="sig_", dir=root)
+    try:
+        t0 = time.perf_counter()
+        sig.store(d)
+        out["store_s"] = round(time.perf_counter() - t0, 3)
+        del sig
+        t0 = time.perf_counter()
+        sig = Signature.load(d)
+        out["load_s"] = round(time.perf_counter() - t0, 3)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    dev = torch.device("cuda:0")
+    times = []
+    for rep in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rp, col, val = sig.to_csr(dev)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        if rep == 0:
+            same = (np.array_equal(rp.cpu().numpy(), A.rowptr) and np.array_equal(col.cpu().numpy(), A.col)
+                    and np.array_equal(val.cpu().numpy(), A.val))
+            log(f"signature -> device CSR equals the generator's CSR: {same}")
+            out["csr_equal"] = bool(same)
+        del rp, col, val
+    out["to_csr_s"] = [round(t, 3) for t in times]
+    t0 = time.perf_counter()
+    r, c, v = sig.to_coo()
+    out["host_decode_s"] = round(time.perf_counter() - t0, 3)
+    del r, c, v
+    log(f"timings: {out}")
+    sample = {}
+    S = workloads.c2_stencil27(40).A
+    srow = np.repeat(np.arange(S.n, dtype=np.uint32), np.diff(S.rowptr))
+    t0 = time.perf_counter()
+    Signature.from_coo(S.n, S.n, srow, S.col.astype(np.uint32), S.val)
+    sample["encode_s"] = round(time.perf_counter() - t0, 3)
+    if oracle.ref_lib() is not None:
+        d = tempfile.mkdtemp(prefix="sigref_", dir=root)
+        try:
+            t0 = time.perf_counter()
+            oracle.ref_sig_coo(d, S.n, S.n, srow, S.col, S.val)
+            sample["reference_partitioner_s"] = round(time.perf_counter() - t0, 3)
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+    log(f"40^3 sample (n={S.n}, nnz={S.nnz}): {sample}")
+    print(json.dumps({"metric": "signature files to device CSR",
+                      "workload": f"27-point stencil {args.grid}^3 (n={A.n}, nnz={A.nnz})",
+                      "nnz_pad": info["nnz_pad"], "row_blocks": info["row_blocks"],
+                      "partitions": info["partitions"], "stream_bytes": sbytes, **out,
+                      "sample_40cubed": sample}), flush=True)
+
+
 def batched_profile(args):
     """Short run for hardware-counter passes on the batched kernels: the batch of --batch-count systems
     x --nrhs right-hand sides, 10 fp64 iterations (set-up plus 10 graph-free iteration launches)."""
@@ -420,7 +496,7 @@ def main():
     ap.add_argument("--inner-tol", type=float, default=0.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", choices=["full", "spmv-profile", "graph", "batched", "autograd", "batched-profile",
-                                       "ingest"],
+                                       "ingest", "signature"],
                     default="full",
                     help="spmv-profile: one 256 MiB device copy (counter calibration) then 10 SpMV launches, "
                          "for rocprofv3 --pmc runs; graph / batched / autograd: the graph-Laplacian, batched "
@@ -437,12 +513,12 @@ def main():
     if args.mode == "spmv-profile":
         spmv_profile(args)
         return
-    if args.mode in ("graph", "batched", "autograd", "batched-profile", "ingest"):
+    if args.mode in ("graph", "batched", "autograd", "batched-profile", "ingest", "signature"):
         import torch
         assert torch.cuda.is_available(), "bench.py needs a GPU"
         torch.cuda.set_device(0)
         {"graph": bench_graph, "batched": bench_batched, "autograd": bench_autograd,
-         "batched-profile": batched_profile, "ingest": bench_ingest}[args.mode](args)
+         "batched-profile": batched_profile, "ingest": bench_ingest, "signature": bench_signature}[args.mode](args)
         return
 
     import torch

@@ -76,6 +76,19 @@ _SIGNATURES = {
     "xjpcg_batch_solve_mixed": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_double,
                                         c_double, c_uint32, c_void_p, c_void_p, POINTER(c_uint32),
                                         POINTER(c_int64), c_void_p]),
+    # xjpcg_sig.h
+    "xjpcg_sig_last_error": (c_char_p, []),
+    "xjpcg_sig_from_coo": (c_int, [POINTER(c_void_p), c_uint32, c_uint32, c_uint32, c_void_p, c_void_p, c_void_p,
+                                   c_int]),
+    "xjpcg_sig_from_cscsym": (c_int, [POINTER(c_void_p), c_uint32, c_uint32, c_void_p, c_void_p, c_void_p, c_int]),
+    "xjpcg_sig_update": (c_int, [c_void_p, c_void_p, c_uint64]),
+    "xjpcg_sig_store": (c_int, [c_void_p, c_char_p]),
+    "xjpcg_sig_load": (c_int, [POINTER(c_void_p), c_char_p]),
+    "xjpcg_sig_destroy": (c_int, [c_void_p]),
+    "xjpcg_sig_info": (c_int, [c_void_p, c_void_p, POINTER(c_uint64), POINTER(c_uint32), POINTER(c_uint32)]),
+    "xjpcg_sig_buffer": (c_int, [c_void_p, c_int, POINTER(c_void_p), POINTER(c_uint64)]),
+    "xjpcg_sig_to_coo": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint64]),
+    "xjpcg_sig_to_csr": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint64, POINTER(c_uint64), c_void_p]),
     # xjpcg_workloads.h
     "xjpcg_stencil_nnz": (c_int64, [c_int, c_int, c_int, c_int]),
     "xjpcg_stencil_fill": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -127,6 +127,13 @@ size_t ingest_scan_temp_bytes(int64_t m) {
     return bytes;
 }
 
+hipError_t ingest_exclusive_sum(void* temp, size_t temp_bytes, const uint32_t* in, uint32_t* out, int64_t m,
+                                hipStream_t s) {
+    if (m <= 0) return hipSuccess;
+    size_t bytes = temp_bytes;
+    return hipcub::DeviceScan::ExclusiveSum(temp, bytes, in, out, (int)m, s);
+}
+
 hipError_t ingest_coo_keys(int64_t m, uint32_t n, uint32_t base, const uint32_t* row, const uint32_t* col,
                            uint64_t* key, int32_t* idx, hipStream_t s) {
     if (m == 0) return hipSuccess;

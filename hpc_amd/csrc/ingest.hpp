@@ -66,5 +66,8 @@ hipError_t ingest_sort_build(int64_t m, uint32_t n, uint64_t* key, int32_t* idx,
                              hipStream_t s);
 size_t ingest_sort_temp_bytes(int64_t m, uint32_t n);
 size_t ingest_scan_temp_bytes(int64_t m);
+// out = exclusive prefix sum of in[0..m) (temp sized by ingest_scan_temp_bytes)
+hipError_t ingest_exclusive_sum(void* temp, size_t temp_bytes, const uint32_t* in, uint32_t* out, int64_t m,
+                                hipStream_t s);
 
 }  // namespace xj

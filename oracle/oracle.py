@@ -65,6 +65,9 @@ def ref_lib():
         R.ref_record_decode.argtypes = [v, v, v, v]
         R.ref_load_coo.argtypes = [u32, u32, v, v, i, v, v]
         R.ref_cscsym_to_coo.argtypes = [u32, u32, v, v, v, i, v, v, v]
+        cs = ctypes.c_char_p
+        R.ref_sig_coo.argtypes = [u32, u32, u32, v, v, v, v, i, cs, cs, i]
+        R.ref_sig_cscsym.argtypes = [u32, u32, v, v, v, v, i, cs, cs, i]
         _R = R
     return _R
 
@@ -160,3 +163,98 @@ def jpcg(rowptr, col, val, diag, b, max_iter, tol, threads=1, records=False):
     if records:
         out["records"] = rec.view(RECORD_DTYPE)
     return out
+
+
+# ---------------------------------------------------------------- signature files
+#
+# The reference prepares matrices for its FPGA SpMV as "signature" files
+# (L2/sparse/src/sw/c++/gen_signature.cpp:58-81).  ref_sig_* run the
+# reference's own partitioner (oracle/_ref); sig_decode restates the
+# reference's own decoder in numpy.
+
+SIG_FILES = ["info.dat", "rbParam.dat", "parParam.dat"] + [f"nnzVal_{c}.dat" for c in range(16)]
+
+
+def ref_sig_coo(directory, m, n, row, col, val, val2=None, base=0):
+    """Reference SpmPar<double>::partitionCooMat (+ updateMat with val2), files written by its Signature::store_*."""
+    R = ref_lib()
+    row = np.ascontiguousarray(row, np.uint32)
+    col = np.ascontiguousarray(col, np.uint32)
+    val = np.ascontiguousarray(val, np.float64)
+    val2 = None if val2 is None else np.ascontiguousarray(val2, np.float64)
+    err = ctypes.create_string_buffer(512)
+    rc = R.ref_sig_coo(m, n, row.size, _p(row), _p(col), _p(val), _p(val2), base, os.fsencode(directory), err, 512)
+    if rc != 0:
+        raise RuntimeError(err.value.decode(errors="replace"))
+
+
+def ref_sig_cscsym(directory, n, row_idx, col_ptr, val, val2=None, base=0):
+    """Reference SpmPar<double>::partitionCscSymMat (+ updateCscSymMat with val2)."""
+    R = ref_lib()
+    row_idx = np.ascontiguousarray(row_idx, np.int64)
+    col_ptr = np.ascontiguousarray(col_ptr, np.int64)
+    val = np.ascontiguousarray(val, np.float64)
+    val2 = None if val2 is None else np.ascontiguousarray(val2, np.float64)
+    full = 2 * row_idx.size - n
+    err = ctypes.create_string_buffer(512)
+    rc = R.ref_sig_cscsym(n, full, _p(row_idx), _p(col_ptr), _p(val), _p(val2), base, os.fsencode(directory), err,
+                          512)
+    if rc != 0:
+        raise RuntimeError(err.value.decode(errors="replace"))
+
+
+def read_sig_files(directory):
+    """{file name: bytes} of a signature directory."""
+    out = {}
+    for nm in SIG_FILES:
+        with open(os.path.join(directory, nm), "rb") as f:
+            out[nm] = f.read()
+    return out
+
+
+def sig_decode(files):
+    """Entries (row, col, val) of signature files ({name: bytes}), channel by channel in stream order.
+
+    Restates the reference decoder: L2/sparse/tests/fp64/spmv/python/signature.py:373-409
+    (create_spm: per row block and channel, start row = block row + channel row
+    offset; per partition, start 4-column block = partition block + channel block
+    offset; values equal to 0 dropped) and matrix_params.py:300-323 (get_chPar: a
+    row-index word every 512 entries, one row per 32; a column word every 64, one
+    4-column block per 4 entries; a 4-double value word per 4 entries).  Small
+    inputs only (a Python loop over words)."""
+    W = 32
+    u32 = lambda b, o: int(np.frombuffer(b, np.uint32, 1, o)[0])  # noqa: E731
+    u16 = lambda b, o: int(np.frombuffer(b, np.uint16, 1, o)[0])  # noqa: E731
+    rb, par = files["rbParam.dat"], files["parParam.dat"]
+    n_rb = u32(rb, 4)
+    rows, cols, vals = [], [], []
+    lane = np.arange(4, dtype=np.int64)
+    for c in range(16):
+        st = files[f"nnzVal_{c}.dat"]
+        off = W
+        p0 = 0
+        for r in range(n_rb):
+            rbo = W + r * 8 * W
+            row0 = u32(rb, rbo) + u16(rb, rbo + 2 * W + 2 * c)
+            npars = u32(rb, rbo + 12)
+            for p in range(p0, p0 + npars):
+                po = W + p * 8 * W
+                cnt = u32(par, po + 2 * W + 4 * c)
+                blk0 = u32(par, po + 4 * W) + u16(par, po + 5 * W + 2 * c)
+                for i in range(0, cnt, 4):
+                    if i % 512 == 0:
+                        ridx = np.frombuffer(st, np.uint16, 16, off)
+                        off += W
+                    if i % 64 == 0:
+                        cidx = np.frombuffer(st, np.uint16, 16, off)
+                        off += W
+                    v = np.frombuffer(st, np.float64, 4, off)
+                    off += W
+                    keep = v != 0
+                    rows.append(np.full(int(keep.sum()), int(ridx[(i % 512) // 32]) + row0, np.int64))
+                    cols.append(((int(cidx[(i % 64) // 4]) + blk0) * 4 + lane)[keep])
+                    vals.append(v[keep])
+            p0 += npars
+        assert off == len(st), f"channel {c}: decoded {off} of {len(st)} bytes"
+    cat = lambda a, t: np.concatenate(a).astype(t) if a else np.zeros(0, t)  # noqa: E731
+    return cat(rows, np.uint32), cat(cols, np.uint32), cat(vals, np.float64)

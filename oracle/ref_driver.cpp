@@ -10,6 +10,10 @@
 //   CGSolverInstr::store/load         pcg/hw/include/impl/cgInstr.hpp:82-156
 //   SparseMatrix::loadCoo/loadCscSym  L2/sparse/include/sw/fp64/matrix_params.hpp:51-134
 //   SpmPar::getCooDatFromCscSym       L2/sparse/include/sw/fp64/gen_signature.hpp:88-112
+//   SpmPar::partitionCooMat / partitionCscSymMat / updateMat
+//                                     L2/sparse/include/sw/fp64/gen_signature.hpp:41-69
+//   Signature::store_rbParam/store_parParam/store_nnz/store_info
+//                                     L2/sparse/include/sw/fp64/signature.hpp:557-573
 #include <algorithm>
 #include <array>
 #include <cassert>
@@ -132,6 +136,59 @@ int ref_cscsym_to_coo(uint32_t n, uint32_t nnz_full, const int64_t* row_idx, con
         std::memcpy(val_out, v.data(), sizeof(double) * nnz_full);
         return 0;
     } catch (...) {
+        return -1;
+    }
+}
+
+// Signature files written by the reference partitioner with the PCG build's
+// geometry (4 values per word, 8-word accumulation, 16 channels, 4096 x 4096
+// partitions, 256-bit words) into dir.  val2, when not null, is applied with
+// the value-only update (updateMat / updateCscSymMat) before the files are
+// written.  Returns 0, -1 on exception (message in err, err_len bytes).
+static void ref_store_sig(xf::sparse::SpmPar<double>& par, const std::string& dir) {
+    xf::sparse::Signature& sg = par.m_sig;
+    sg.store_rbParam(dir + "/rbParam.dat");
+    sg.store_parParam(dir + "/parParam.dat");
+    std::string names[16];
+    for (int i = 0; i < 16; ++i) names[i] = dir + "/nnzVal_" + std::to_string(i) + ".dat";
+    sg.store_nnz(names);
+    sg.store_info(dir + "/info.dat");
+}
+
+static void ref_copy_err(const char* what, char* err, int err_len) {
+    if (err && err_len > 0) {
+        std::strncpy(err, what, (size_t)err_len - 1);
+        err[err_len - 1] = 0;
+    }
+}
+
+int ref_sig_coo(uint32_t m, uint32_t n, uint32_t nnz, const uint32_t* row, const uint32_t* col, const double* val,
+                const double* val2, int base, const char* dir, char* err, int err_len) {
+    try {
+        xf::sparse::SpmPar<double> par(4, 8, 16, 4096, 4096, 256);
+        par.partitionCooMat(m, n, nnz, row, col, val, base);
+        if (val2) {
+            par.checkUpdateDim(m, n, nnz);
+            par.updateMat(val2);
+        }
+        ref_store_sig(par, dir);
+        return 0;
+    } catch (const std::exception& e) {
+        ref_copy_err(e.what(), err, err_len);
+        return -1;
+    }
+}
+
+int ref_sig_cscsym(uint32_t n, uint32_t nnz_full, const int64_t* row_idx, const int64_t* col_ptr, const double* val,
+                   const double* val2, int base, const char* dir, char* err, int err_len) {
+    try {
+        xf::sparse::SpmPar<double> par(4, 8, 16, 4096, 4096, 256);
+        par.partitionCscSymMat<int64_t>(n, nnz_full, row_idx, col_ptr, val, base);
+        if (val2) par.updateCscSymMat<int64_t>(n, nnz_full, row_idx, col_ptr, val2, base);
+        ref_store_sig(par, dir);
+        return 0;
+    } catch (const std::exception& e) {
+        ref_copy_err(e.what(), err, err_len);
         return -1;
     }
 }
