@@ -1,8 +1,8 @@
 // Signature streams decoded on the GPU (the host only builds the per-segment
-// table): one thread per value word finds its segment by binary search on the
-// value-word prefix, locates its row-index and column words from the fixed
-// interleave (a row word per 128 value words, a column word per 16), and
-// turns its four lanes into (key = row*n + col, position, value) triples for
+// table): one thread per value word finds its segment (one binary search on
+// the value-word prefix per block, then a short walk per thread), locates its
+// row-index and column words from the fixed interleave (a row word per 128
+// value words, a column word per 16), and turns its four lanes into (key = row*n + col, position, value) triples for
 // the stable radix sort that builds the CSR.  Zero lanes (padding, and stored
 // zeros, which the format cannot tell apart) are dropped via count + scan.
 //
@@ -30,14 +30,22 @@ __device__ __forceinline__ uint32_t find_segment(const Segment* __restrict__ seg
     return lo;
 }
 
+// Segment of value word w (w >= w0), walking forward from s0 = segment of w0:
+// the block searches once for its first word, and segments are long (a
+// padded row is 8 value words at least), so a thread takes a step or two.
+__device__ __forceinline__ uint32_t walk_segment(const Segment* __restrict__ seg, uint32_t nseg, uint32_t s0,
+                                                 uint32_t w) {
+    uint32_t s = s0;
+    while (s + 1 < nseg && seg[s + 1].vw0 <= w) ++s;
+    return s;
+}
+
 struct WordView {
     uint32_t row, col0;
     double v[4];
 };
 
-__device__ __forceinline__ WordView view_word(const Segment* __restrict__ seg, uint32_t nseg,
-                                              const uint8_t* __restrict__ words, uint32_t w) {
-    const Segment S = seg[find_segment(seg, nseg, w)];
+__device__ __forceinline__ WordView view_word(const Segment& S, const uint8_t* __restrict__ words, uint32_t w) {
     const uint32_t q = w - S.vw0;
     const uint64_t vpos = S.word0 + q + (q >> 4) + 1 + (q >> 7) + 1;
     const uint64_t rpos = S.word0 + 137ull * (q >> 7);
@@ -58,8 +66,11 @@ __device__ __forceinline__ WordView view_word(const Segment* __restrict__ seg, u
 
 __global__ __launch_bounds__(kBlock) void sig_count(uint64_t vw, const Segment* __restrict__ seg, uint32_t nseg,
                                                     const uint8_t* __restrict__ words, uint32_t* __restrict__ cnt) {
-    for (uint64_t w = (uint64_t)blockIdx.x * kBlock + threadIdx.x; w < vw; w += (uint64_t)gridDim.x * kBlock) {
-        const Segment S = seg[find_segment(seg, nseg, (uint32_t)w)];
+    for (uint64_t w0 = (uint64_t)blockIdx.x * kBlock; w0 < vw; w0 += (uint64_t)gridDim.x * kBlock) {
+        const uint32_t s0 = find_segment(seg, nseg, (uint32_t)w0);
+        const uint64_t w = w0 + threadIdx.x;
+        if (w >= vw) continue;
+        const Segment S = seg[walk_segment(seg, nseg, s0, (uint32_t)w)];
         const uint32_t q = (uint32_t)w - S.vw0;
         const uint64_t vpos = S.word0 + q + (q >> 4) + 1 + (q >> 7) + 1;
         const double2* p = reinterpret_cast<const double2*>(words + vpos * kWordBytes);
@@ -73,8 +84,11 @@ __global__ __launch_bounds__(kBlock) void sig_emit(uint64_t vw, const Segment* _
                                                    uint32_t m, uint32_t n, uint64_t* __restrict__ key,
                                                    int32_t* __restrict__ idx, double* __restrict__ val,
                                                    int32_t* __restrict__ bad) {
-    for (uint64_t w = (uint64_t)blockIdx.x * kBlock + threadIdx.x; w < vw; w += (uint64_t)gridDim.x * kBlock) {
-        const WordView V = view_word(seg, nseg, words, (uint32_t)w);
+    for (uint64_t w0 = (uint64_t)blockIdx.x * kBlock; w0 < vw; w0 += (uint64_t)gridDim.x * kBlock) {
+        const uint32_t s0 = find_segment(seg, nseg, (uint32_t)w0);
+        const uint64_t w = w0 + threadIdx.x;
+        if (w >= vw) continue;
+        const WordView V = view_word(seg[walk_segment(seg, nseg, s0, (uint32_t)w)], words, (uint32_t)w);
         uint32_t p = pos[w];
         for (int j = 0; j < 4; ++j) {
             if (V.v[j] == 0.0) continue;
