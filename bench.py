@@ -507,8 +507,8 @@ def main():
     ap.add_argument("--nrhs", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; the multi-GPU row split is a design "
-                                   "note (DESIGN.md), run with --gpus 1"}))
+        print(json.dumps({"error": "one GPU only: the row-block split over several GPUs is designed, not "
+                                   "built; run with --gpus 1"}))
         sys.exit(2)
     if args.mode == "spmv-profile":
         spmv_profile(args)
