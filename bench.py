@@ -61,9 +61,11 @@ def cpu_baseline(A, budget_s=15.0):
     return out, threads_all
 
 
-LAYOUT_NAMES = {1: "tiles", 2: "panels", 3: "patterns"}
-LAYOUT_KERNELS = {1: "spmv_tiles<double, %s columns>", 2: "spmv_panels<double>", 3: "spmv_pattern<double>"}
-TRAFFIC_PROFILES = {"tiles": "spmv_tiles_stencil27_160.md", "patterns": "spmv_pattern_stencil27_160.md"}
+LAYOUT_NAMES = {1: "tiles", 2: "panels", 3: "patterns", 4: "diagonals"}
+LAYOUT_KERNELS = {1: "spmv_tiles<double, %s columns>", 2: "spmv_panels<double>", 3: "spmv_pattern<double>",
+                  4: "spmv_dia<double>"}
+TRAFFIC_PROFILES = {"tiles": "spmv_tiles_stencil27_160.md", "patterns": "spmv_pattern_stencil27_160.md",
+                    "diagonals": "spmv_dia_stencil27_160.md"}
 
 
 def measured_traffic(grid, layout_name, compact=True):
@@ -551,7 +553,7 @@ def main():
     stream = torch.cuda.current_stream()
 
     # ---- SpMV: per-launch HIP events on the launch stream, every layout that applies; the reported one is
-    # the layout the operator chose for itself at creation (the one its solves use)
+    # the layout the operator chose for its standalone SpMV at creation (its solves use their own choice)
     xs = torch.ones(A.n, dtype=torch.float64, device=dev)
     ys = torch.empty_like(xs)
     nb = spmv_bytes(A.n, A.nnz)
@@ -570,7 +572,7 @@ def main():
         return np.array([e0.elapsed_time(e1) for e0, e1 in evs])
 
     layouts = {}
-    for lay in (1, 3):
+    for lay in (1, 3, 4):
         try:
             op.set_layout(lay)
         except Exception:
@@ -597,7 +599,7 @@ def main():
         return e0.elapsed_time(e1) / args.spmv_repeats
 
     inner_layouts = {}
-    for lay in (1, 3):
+    for lay in (1, 3, 4):
         try:
             inner.set_layout(lay)
         except Exception:
@@ -632,8 +634,9 @@ def main():
     true_res64 = (rr / bb) ** 0.5
     fp64_ms = float(np.median(times))
     it_bytes = iter_bytes(A.n, A.nnz) * info.executed
-    log(f"JPCG fp64: {fp64_ms:.1f} ms, {info.executed} iterations, recursive res {info.residual:.2e}, "
-        f"true res {true_res64:.2e}, {it_bytes / (fp64_ms * 1e-3) / 1e9:.0f} GB/s (B_iter formula)")
+    log(f"JPCG fp64 ({LAYOUT_NAMES[op.solve_layout]}): {fp64_ms:.1f} ms, {info.executed} iterations, recursive "
+        f"res {info.residual:.2e}, true res {true_res64:.2e}, {it_bytes / (fp64_ms * 1e-3) / 1e9:.0f} GB/s "
+        f"(B_iter formula)")
 
     # ---- JPCG mixed-refined
     xm = torch.empty_like(b)
@@ -691,11 +694,11 @@ def main():
                  "fp32_values_GBs": round(nb32 / (spmv32_ms * 1e-3) / 1e9, 1)},
         "jpcg": {
             "fp64": {"time_to_tol_ms": round(fp64_ms, 3), "wall_ms": round(t_fp64_wall * 1e3, 3),
-                     "layout": LAYOUT_NAMES[chosen],
+                     "layout": LAYOUT_NAMES[op.solve_layout],
                      "iterations": info.executed, "recursive_res": info.residual, "true_res": true_res64,
                      "iter_GBs_formula": round(it_bytes / (fp64_ms * 1e-3) / 1e9, 1)},
             "mixed": {"time_to_tol_ms": round(mixed_ms, 3), "outer": minfo.outer, "inner_total": minfo.inner_total,
-                      "inner_layout": LAYOUT_NAMES[inner.layout],
+                      "inner_layout": LAYOUT_NAMES[inner.solve_layout],
                       "inner_tol": args.inner_tol, "true_res": true_res_m, "rel_diff_vs_fp64": err_vs64},
             "speedup_mixed_vs_fp64": round(fp64_ms / mixed_ms, 3),
         },

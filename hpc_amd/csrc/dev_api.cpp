@@ -135,6 +135,13 @@ XJPCG_Status_t xjpcg_op_set_layout(xjpcg_op_t* op, int layout, void* stream, int
     });
 }
 
+XJPCG_Status_t xjpcg_op_solve_layout(const xjpcg_op_t* op, int* h_layout) {
+    return guarded([&] {
+        const xj::Operator& o = impl_of(const_cast<xjpcg_op_t*>(op));
+        if (h_layout) *h_layout = o.solve_layout();
+    });
+}
+
 XJPCG_Status_t xjpcg_op_info(const xjpcg_op_t* op, int32_t* n, int64_t* nnz, int32_t* ntiles, int32_t* grid) {
     return guarded([&] {
         const xj::Operator& o = impl_of(const_cast<xjpcg_op_t*>(op));

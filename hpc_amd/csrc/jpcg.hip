@@ -11,6 +11,7 @@
 //  gather_values     KEEP_NZ_LAYOUT value refresh through the stored permutation
 //  sddmm_tiles       g_k = -lam[row_k] * x[col_k] on the sparsity pattern
 //  spmv_pattern / cg_spmv_pattern SpMV on column-offset pattern tiles (stencils)
+//  spmv_dia / cg_spmv_dia         SpMV on the diagonal layout (few global offsets)
 //  spmv_panels / cg_spmv_panels   SpMV on the panel-swept layout (operators
 //                    whose columns scatter over x), standalone and in the CG loop
 //
@@ -555,6 +556,122 @@ __global__ __launch_bounds__(kBlock) void spmv_pattern(int ntiles, int nnz, cons
     }
 }
 
+// --------------------------------------------------------------------------
+// diagonal layout
+
+// Row i of A times x on the diagonal layout, sixteen diagonals at a time:
+// the zero-padded value of each diagonal and x at the entry's column
+// (clamped into [0, n) for a diagonal the row does not have) are loaded
+// together with the mask, then the products of the row's entries are added
+// in ascending offset order, each rounded before it is added, left to right
+// from dsplit[i]*x[i] (or 0) -- the arithmetic of the row tiles.  Value loads
+// and the x reads of one offset are contiguous across the lanes of a wave.
+template <typename VT>
+__device__ __forceinline__ double dia_row(const DiaDev& D, int n, int i, const double* __restrict__ dsplit,
+                                          const double* __restrict__ x) {
+#pragma clang fp contract(off)
+    const VT* __restrict__ val = static_cast<const VT*>(D.val);
+    const int* __restrict__ off = D.off;
+    const int nd = D.ndia;
+    const uint32_t m = D.mask[i];
+    double s = dsplit ? dsplit[i] * x[i] : 0.0;
+#pragma nounroll
+    for (int d0 = 0; d0 < nd; d0 += 16) {
+        VT v[16];
+        double g[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int d = d0 + u;
+            if (d < nd) {
+                v[u] = ld_stream(val + (size_t)d * n + i);
+                int c = i + off[d];
+                c = c < 0 ? 0 : (c >= n ? n - 1 : c);
+                g[u] = x[c];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int d = d0 + u;
+            if (d < nd && ((m >> d) & 1u)) s += (double)v[u] * g[u];
+        }
+    }
+    return s;
+}
+
+// y = A x (or b - A x), one lane per row, 256 consecutive rows per workgroup
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void spmv_dia(int n, DiaDev D, const double* __restrict__ dsplit,
+                                                   const double* __restrict__ x, double* __restrict__ y,
+                                                   const double* __restrict__ b) {
+    const int i = xcd_remap(blockIdx.x, gridDim.x) * kBlock + (int)threadIdx.x;
+    if (i >= n) return;
+    const double s = dia_row<VT>(D, n, i, dsplit, x);
+    y[i] = b ? b[i] - s : s;
+}
+
+// CG SpMV on the diagonal layout: kDiaCgRows consecutive rows per workgroup,
+// lane l taking rows l, l + 256, ... of the block (every address follows from
+// the block index, so nothing waits on a table); Ap is stored and each
+// wavefront leaves one p.Ap partial, its rows' products summed in a fixed
+// order (pass by pass, xor-butterfly within the wavefront) -- no barrier.
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void cg_spmv_dia(CgParams P) {
+    CgState* st = P.st;
+    const int stop = st->stop;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    if (stop) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->k = st->k + 1;  // no other workgroup of this launch reads k
+    double pap = 0.0;
+#pragma unroll
+    for (int q = 0; q < kDiaCgRows / kBlock; ++q) {
+        const int i = b * kDiaCgRows + q * kBlock + (int)threadIdx.x;
+        if (i < P.n) {
+            const double pi = P.p[i];
+            const double y = dia_row<VT>(P.dia, P.n, i, P.dsplit, P.p);
+            P.Ap[i] = y;
+            pap += pi * y;
+        }
+    }
+    pap = wave_sum(pap);
+    if ((threadIdx.x & 63) == 0) P.partA[b * kWaves + (threadIdx.x >> 6)] = pap;
+}
+
+// Diagonal values and masks from the CSR, one lane per row: entry k of row i
+// lands in diagonal d with off[d] = col[k] - i.  dval is zeroed beforehand
+// (a refresh of the values rewrites the same slots).
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void fill_dia(int n, const int* __restrict__ rowptr, const int* __restrict__ col,
+                                                   const VT* __restrict__ val, int ndia, const int* __restrict__ off,
+                                                   VT* __restrict__ dval, uint32_t* __restrict__ dmask,
+                                                   int* __restrict__ fail) {
+    __shared__ int loff[kDiaMax];
+    if ((int)threadIdx.x < kDiaMax) loff[threadIdx.x] = (int)threadIdx.x < ndia ? off[threadIdx.x] : INT32_MAX;
+    __syncthreads();
+    for (int i = blockIdx.x * kBlock + (int)threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        uint32_t m = 0;
+        int prev = -1;
+        bool bad = ndia <= 0 && rowptr[i + 1] > rowptr[i];
+        for (int k = rowptr[i]; k < rowptr[i + 1] && !bad; ++k) {
+            const int dlt = col[k] - i;
+            int lo = 0, hi = ndia - 1;
+            while (lo < hi) {  // first slot holding an offset >= dlt
+                const int mid = (lo + hi) >> 1;
+                if (loff[mid] < dlt) lo = mid + 1;
+                else hi = mid;
+            }
+            if (loff[lo] != dlt || lo <= prev) {
+                bad = true;
+                break;
+            }
+            prev = lo;
+            m |= 1u << lo;
+            dval[(size_t)lo * n + i] = val[k];
+        }
+        dmask[i] = m;
+        if (bad) atomicOr(fail, 1);
+    }
+}
+
 // Column-offset pattern plan on the GPU, one workgroup per tile (the same
 // plan as the host planner plan_patterns): lane t checks row t (at most 32
 // entries, columns strictly ascending and inside [0, n)) and inserts its
@@ -565,7 +682,8 @@ __global__ __launch_bounds__(kBlock) void spmv_pattern(int ntiles, int nnz, cons
 __global__ __launch_bounds__(kBlock) void plan_pattern_tiles(int n, int ntiles, const int* __restrict__ tile_row,
                                                              const int* __restrict__ rowptr,
                                                              const int* __restrict__ col, uint32_t* __restrict__ pmask,
-                                                             int* __restrict__ poffs, int* __restrict__ fail) {
+                                                             int* __restrict__ poffs, int unused,
+                                                             int* __restrict__ fail) {
     constexpr int kSlots = 64;
     constexpr int kEmpty = INT32_MIN;  // no offset col - row reaches it for n < 2^31
     __shared__ int tab[kSlots];
@@ -625,8 +743,8 @@ __global__ __launch_bounds__(kBlock) void plan_pattern_tiles(int n, int ntiles, 
     }
     __syncthreads();
     const int m = cnt;
-    if (tid < 32) poffs[t * 32 + tid] = tid < m ? sorted[tid] : 0;
-    if (mine) {
+    if (tid < 32) poffs[t * 32 + tid] = tid < m ? sorted[tid] : unused;
+    if (mine && pmask) {
         uint32_t mask = 0;
         for (int k = a; k < b; ++k) {
             const int d = col[k] - i;
@@ -693,6 +811,10 @@ __global__ __launch_bounds__(kBlock) void stream_copy(int64_t n, const double* _
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n2; i += (int64_t)gridDim.x * kBlock)
         __builtin_nontemporal_store(__builtin_nontemporal_load(s + i), d + i);
     if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) dst[n - 1] = src[n - 1];
+}
+
+__global__ __launch_bounds__(kBlock) void fill_kernel(int64_t n, double v, double* __restrict__ x) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) x[i] = v;
 }
 
 // x = x + a*d  (iterative-refinement update)
@@ -792,6 +914,12 @@ hipError_t launch_cg_iteration(const CgParams& P, bool f32, bool c16, hipStream_
             cg_spmv_panels<float><<<P.pan.nb, kPanelBlock, lds, s>>>(P);
         else
             cg_spmv_panels<double><<<P.pan.nb, kPanelBlock, lds, s>>>(P);
+    } else if (P.dia_on) {
+        const int g = (P.n + kDiaCgRows - 1) / kDiaCgRows;
+        if (f32)
+            cg_spmv_dia<float><<<g, kBlock, 0, s>>>(P);
+        else
+            cg_spmv_dia<double><<<g, kBlock, 0, s>>>(P);
     } else if (P.pat_on) {
         if (f32)
             cg_spmv_pattern<float><<<pattern_groups(P.tiles), kBlock, 0, s>>>(P);
@@ -813,6 +941,15 @@ hipError_t launch_spmv(const CsrDev& A, const double* x, double* y, const double
             spmv_panels<float><<<A.pan.nb, kPanelBlock, lds, s>>>(A.pan, A.n, A.dsplit, x, y, b);
         else
             spmv_panels<double><<<A.pan.nb, kPanelBlock, lds, s>>>(A.pan, A.n, A.dsplit, x, y, b);
+        return hipGetLastError();
+    }
+    if (A.dia_on) {
+        const int g = (A.n + kBlock - 1) / kBlock;
+        if (g <= 0) return hipSuccess;
+        if (A.val_f32)
+            spmv_dia<float><<<g, kBlock, 0, s>>>(A.n, A.dia, A.dsplit, x, y, b);
+        else
+            spmv_dia<double><<<g, kBlock, 0, s>>>(A.n, A.dia, A.dsplit, x, y, b);
         return hipGetLastError();
     }
     if (A.ntiles <= 0) return hipSuccess;
@@ -841,7 +978,36 @@ hipError_t launch_spmv(const CsrDev& A, const double* x, double* y, const double
 
 hipError_t launch_plan_patterns(const CsrDev& A, uint32_t* pmask, int* poffs, int* fail, hipStream_t s) {
     if (A.ntiles <= 0) return hipSuccess;
-    plan_pattern_tiles<<<A.ntiles, kBlock, 0, s>>>(A.n, A.ntiles, A.tile_row, A.rowptr, A.col, pmask, poffs, fail);
+    plan_pattern_tiles<<<A.ntiles, kBlock, 0, s>>>(A.n, A.ntiles, A.tile_row, A.rowptr, A.col, pmask, poffs, 0, fail);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_offsets(const CsrDev& A, int* toffs, int* fail, hipStream_t s) {
+    if (A.ntiles <= 0) return hipSuccess;
+    plan_pattern_tiles<<<A.ntiles, kBlock, 0, s>>>(A.n, A.ntiles, A.tile_row, A.rowptr, A.col, nullptr, toffs,
+                                                   INT32_MIN, fail);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(int64_t n, double v, double* x, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int64_t g = (n + kBlock - 1) / kBlock;
+    if (g > 65536) g = 65536;
+    fill_kernel<<<(int)g, kBlock, 0, s>>>(n, v, x);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_dia(const CsrDev& A, int ndia, const int* off, void* dval, uint32_t* dmask, int* fail,
+                           hipStream_t s) {
+    if (A.n <= 0) return hipSuccess;
+    int g = (A.n + kBlock - 1) / kBlock;
+    if (g > 65536) g = 65536;
+    if (A.val_f32)
+        fill_dia<float><<<g, kBlock, 0, s>>>(A.n, A.rowptr, A.col, static_cast<const float*>(A.val), ndia, off,
+                                             static_cast<float*>(dval), dmask, fail);
+    else
+        fill_dia<double><<<g, kBlock, 0, s>>>(A.n, A.rowptr, A.col, static_cast<const double*>(A.val), ndia, off,
+                                              static_cast<double*>(dval), dmask, fail);
     return hipGetLastError();
 }
 

@@ -30,6 +30,19 @@ struct PanelDev {
 //   dsplit[n]              optional fp64 diagonal kept outside the CSR
 //                          (the CSR then holds off-diagonal entries only)
 //   tile_row[ntiles+1]     first row of each tile
+// Diagonal layout (stencil-like operators with few distinct offsets): one
+// value array per global column offset col - row, ascending, with 0 where a
+// row has no entry there, and a presence mask per row.
+constexpr int kDiaMax = 32;
+constexpr int kDiaCgRows = 256;     // rows per workgroup of the diagonal layout's CG SpMV (4 partials each)
+constexpr int dia_cg_partials(int n) { return (n + kDiaCgRows - 1) / kDiaCgRows * 4; }
+struct DiaDev {
+    int ndia = 0;
+    const int* off = nullptr;        // [ndia] offsets, ascending
+    const void* val = nullptr;       // [ndia][n] in the operator's value type
+    const uint32_t* mask = nullptr;  // [n]: bit d set when row i has an entry at off[d]
+};
+
 struct CsrDev {
     int n = 0;
     int64_t nnz = 0;
@@ -47,6 +60,8 @@ struct CsrDev {
     const uint32_t* pmask = nullptr;  // column-offset pattern tiles (tile_spmv_pattern): mask per row
     const int* poffs = nullptr;       // 32 offsets per tile
     bool pat_on = false;
+    DiaDev dia;                       // diagonal layout, used by the SpMV kernels when dia_on
+    bool dia_on = false;
 };
 
 struct JpcgParams {
@@ -82,8 +97,9 @@ struct CgState {
     int pad;
 };
 
-// Arguments of the multi-kernel loop: SpMV (one workgroup per tile, or per
-// row block of the panel layout, with one p.Ap partial each), optionally a
+// Arguments of the multi-kernel loop: SpMV (one workgroup per tile or per row
+// block of the panel layout, with one p.Ap partial each; one partial per
+// wavefront of 256-row workgroups on the diagonal layout), optionally a
 // reduction of those partials, then two vector kernels (G2 workgroups,
 // grid-stride).  ntiles counts the SpMV workgroups, hence the partials.
 struct CgParams {
@@ -94,6 +110,8 @@ struct CgParams {
     int pat_on = 0;                      // 1: the SpMV runs on column-offset pattern tiles
     const uint32_t* pmask = nullptr;
     const int* poffs = nullptr;
+    int dia_on = 0;                      // 1: the SpMV runs on the diagonal layout `dia`
+    DiaDev dia;
     int split_alpha = 0;                 // 1: cg_alpha first folds the ntiles partials into nA2
     int nA2 = 0;
     double* partA2 = nullptr;            // nA2 doubles
@@ -127,6 +145,12 @@ hipError_t launch_jpcg(const JpcgParams& P, bool f32, bool c16, int G, hipStream
 hipError_t launch_spmv(const CsrDev& A, const double* x, double* y, const double* b, hipStream_t s);
 hipError_t launch_build_col16(const CsrDev& A, int16_t* col16, int* overflow, hipStream_t s);
 hipError_t launch_plan_patterns(const CsrDev& A, uint32_t* pmask, int* poffs, int* fail, hipStream_t s);
+// per-tile ascending offsets as for the pattern plan, unused slots = INT32_MIN, no masks
+hipError_t launch_tile_offsets(const CsrDev& A, int* toffs, int* fail, hipStream_t s);
+// diagonal values and masks from the CSR (dval zeroed beforehand); *fail when
+// a row's offset is not in `off` or its offsets do not ascend
+hipError_t launch_fill_dia(const CsrDev& A, int ndia, const int* off, void* dval, uint32_t* dmask, int* fail,
+                           hipStream_t s);
 int dot_blocks(int n);
 hipError_t launch_dot(int n, const double* a, const double* b, double* part, double* out, hipStream_t s);
 hipError_t launch_gather_f64(int64_t nnz, const int64_t* perm, const double* src, double* dst, hipStream_t s);
@@ -134,6 +158,7 @@ hipError_t launch_gather_f32(int64_t nnz, const int64_t* perm, const double* src
 // dst[k] = src[perm[k]] in the operator's value type (fp32 when f32)
 hipError_t launch_gather_vals(int64_t nnz, const int64_t* perm, const void* src, void* dst, bool f32, hipStream_t s);
 hipError_t launch_axpy(int n, double a, const double* d, double* x, hipStream_t s);
+hipError_t launch_fill(int64_t n, double v, double* x, hipStream_t s);
 hipError_t launch_stream_copy(int64_t n, const double* src, double* dst, hipStream_t s);
 hipError_t launch_sddmm(const CsrDev& A, const double* lam, const double* x, double* g, hipStream_t s);
 

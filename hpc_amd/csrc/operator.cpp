@@ -2,6 +2,7 @@
 #include "operator.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -220,32 +221,68 @@ Operator::Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t*
             A_.col16 = col16_.ptr;
         }
     }
-    // Layout choice, timed on this operator: column-offset pattern tiles when
-    // every tile has at most 32 distinct offsets (stencils), and, when the
-    // columns are too spread for the int16 stream and x is far larger than an
-    // XCD's L2, the panel layout.  The faster layout stays (panels only when
-    // at least 15% faster than the tiles).
+    // Layout choices, timed on this operator: column-offset pattern tiles when
+    // every tile has at most 32 distinct offsets (stencils), the diagonal
+    // layout when the whole matrix has at most 32 (and its value arrays stay
+    // within twice the stored entries), and, when the columns are too spread
+    // for the int16 stream and x is far larger than an XCD's L2, the panel
+    // layout (only when at least 15% faster than the tiles).  The standalone
+    // SpMV keeps the layout with the fastest SpMV launches; the solves keep
+    // the one with the fastest short CG run among those, which can differ
+    // (a CG iteration adds the Ap store, p.Ap partials and the vector kernels).
     if (nnz > 0) {
+        std::vector<int> cand(1, kTiles);
         const float t_tiles = time_layout(kTiles, s);
         float best = t_tiles;
         if (build_patterns(s)) {
+            cand.push_back(kPatterns);
             const float t_pat = time_layout(kPatterns, s);
             if (t_pat > 0.0f && t_pat < best) {
                 best = t_pat;
                 auto_layout_ = kPatterns;
             }
         }
-        if (!has_col16_ && n >= kPanelAutoRows && build_panels(s)) {
-            const float t_pan = time_layout(kPanels, s);
-            if (t_pan > 0.0f && t_pan < 0.85f * t_tiles && t_pan < best) {
-                best = t_pan;
-                auto_layout_ = kPanels;
+        if (build_diagonals(s)) {
+            cand.push_back(kDiagonals);
+            const float t_dia = time_layout(kDiagonals, s);
+            if (t_dia > 0.0f && t_dia < best) {
+                best = t_dia;
+                auto_layout_ = kDiagonals;
             }
         }
-        if (auto_layout_ != kPanels) free_panels();
-        if (auto_layout_ != kPatterns) free_patterns();
+        if (!has_col16_ && n >= kPanelAutoRows && build_panels(s)) {
+            const float t_pan = time_layout(kPanels, s);
+            if (t_pan > 0.0f && t_pan < 0.85f * t_tiles) {
+                cand.push_back(kPanels);
+                if (t_pan < best) {
+                    best = t_pan;
+                    auto_layout_ = kPanels;
+                }
+            }
+        }
+        auto_solve_ = auto_layout_;
+        if (cand.size() > 1) {
+            DevBuf<double> tmp;
+            tmp.ensure((size_t)3 * n);
+            hip_check(launch_fill(3 * (int64_t)n, 1.0, tmp.ptr, s), "timing vectors");
+            double tbest = 0.0;
+            for (int L : cand) {
+                const double t = time_solve_layout(L, tmp.ptr, s);
+                if (tbest == 0.0 || t < tbest) {
+                    tbest = t;
+                    auto_solve_ = L;
+                }
+            }
+            drop_graph();  // captured with the timing vectors
+        }
+        const auto used = [&](int L) { return auto_layout_ == L || auto_solve_ == L; };
+        if (!used(kPanels)) free_panels();
+        if (!used(kPatterns)) free_patterns();
+        if (!used(kDiagonals)) free_diagonals();
         A_.pan_on = auto_layout_ == kPanels;
         A_.pat_on = auto_layout_ == kPatterns;
+        A_.dia_on = auto_layout_ == kDiagonals;
+        solve_layout_ = auto_solve_;
     }
     hip_check(persistent_grid(val_f32, A_.col16 != nullptr, &G_), "persistent kernel occupancy");
     part_.ensure((size_t)5 * G_ + 4096);
@@ -301,6 +338,95 @@ void Operator::free_patterns() {
     A_.pat_on = false;
     has_pat_ = false;
     pat_tried_ = false;
+}
+
+// Offsets from the per-tile plan (on the GPU), their union on the host, then
+// the value arrays and masks on the GPU.  Refused (false) for more than 32
+// global offsets, rows whose columns do not ascend, or value arrays beyond
+// twice the stored entries.
+bool Operator::build_diagonals(hipStream_t s) {
+    if (has_dia_) return true;
+    if (dia_tried_ || A_.nnz <= 0 || A_.ntiles <= 0) return false;
+    dia_tried_ = true;
+    std::vector<int32_t> offs;
+    {
+        DevBuf<int32_t> toffs;
+        try {
+            toffs.ensure((size_t)A_.ntiles * 32);
+        } catch (const Error& e) {
+            if (e.status != XJPCG_STATUS_ALLOC_FAILED) throw;
+            return false;
+        }
+        hip_check(hipMemsetAsync(abort_.ptr, 0, sizeof(int), s), "reset flag");
+        hip_check(launch_tile_offsets(A_, toffs.ptr, abort_.ptr, s), "tile offsets");
+        std::vector<int32_t> h((size_t)A_.ntiles * 32);
+        int fail = 1;
+        hip_check(hipMemcpyAsync(h.data(), toffs.ptr, sizeof(int32_t) * h.size(), hipMemcpyDeviceToHost, s),
+                  "tile offsets");
+        hip_check(hipMemcpyAsync(&fail, abort_.ptr, sizeof(int), hipMemcpyDeviceToHost, s), "tile offsets");
+        hip_check(hipStreamSynchronize(s), "tile offsets");
+        if (fail) return false;
+        for (int32_t v : h)
+            if (v != INT32_MIN) offs.push_back(v);
+        std::sort(offs.begin(), offs.end());
+        offs.erase(std::unique(offs.begin(), offs.end()), offs.end());
+    }
+    const int ndia = (int)offs.size();
+    if (ndia == 0 || ndia > kDiaMax || (int64_t)ndia * A_.n > 2 * A_.nnz) return false;
+    const int nblk = dia_cg_partials(A_.n);  // the CG kernel's p.Ap partials
+    const size_t vbytes = (size_t)ndia * A_.n * (A_.val_f32 ? 4 : 8);
+    try {
+        doff_.ensure(kDiaMax + 8);  // padded: lanes never read past the table
+        dmask_.ensure((size_t)A_.n);
+        dval_.ensure((vbytes + 7) / 8);
+        partA_.ensure((size_t)std::max({A_.ntiles, nblk, A_.pan.nb}));
+    } catch (const Error& e) {
+        if (e.status != XJPCG_STATUS_ALLOC_FAILED) throw;
+        free_diagonals();
+        dia_tried_ = true;
+        return false;
+    }
+    offs.resize(kDiaMax + 8, 0);
+    hip_check(hipMemcpyAsync(doff_.ptr, offs.data(), sizeof(int32_t) * offs.size(), hipMemcpyHostToDevice, s),
+              "upload offsets");
+    hip_check(hipMemsetAsync(dval_.ptr, 0, vbytes, s), "clear diagonals");
+    hip_check(hipMemsetAsync(abort_.ptr, 0, sizeof(int), s), "reset flag");
+    hip_check(launch_fill_dia(A_, ndia, doff_.ptr, dval_.ptr, dmask_.ptr, abort_.ptr, s), "fill diagonals");
+    int fail = 1;
+    hip_check(hipMemcpyAsync(&fail, abort_.ptr, sizeof(int), hipMemcpyDeviceToHost, s), "fill diagonals");
+    hip_check(hipStreamSynchronize(s), "fill diagonals");  // host tables end with this scope
+    if (fail) {
+        free_diagonals();
+        dia_tried_ = true;
+        return false;
+    }
+    A_.dia.ndia = ndia;
+    A_.dia.off = doff_.ptr;
+    A_.dia.val = dval_.ptr;
+    A_.dia.mask = dmask_.ptr;
+    has_dia_ = true;
+    dia_stale_ = false;
+    return true;
+}
+
+void Operator::free_diagonals() {
+    doff_.release();
+    dmask_.release();
+    dval_.release();
+    A_.dia = DiaDev();
+    A_.dia_on = false;
+    has_dia_ = false;
+    dia_tried_ = false;
+}
+
+void Operator::refresh_copies(hipStream_t s) const {
+    refresh_panels(s);
+    if (!(A_.dia_on || solve_layout_ == kDiagonals) || !dia_stale_) return;
+    // same pattern: the fill rewrites every stored slot, the zeros stay
+    hip_check(hipMemsetAsync(abort_.ptr, 0, sizeof(int), s), "reset flag");
+    hip_check(launch_fill_dia(A_, A_.dia.ndia, A_.dia.off, const_cast<void*>(A_.dia.val), dmask_.ptr, abort_.ptr, s),
+              "diagonal values");
+    dia_stale_ = false;
 }
 
 bool Operator::build_panels(hipStream_t s, const int32_t* rp_host, const int32_t* col_host) {
@@ -359,16 +485,34 @@ void Operator::free_panels() {
 }
 
 void Operator::refresh_panels(hipStream_t s) const {
-    if (!A_.pan_on || !pan_stale_) return;
+    if (!(A_.pan_on || solve_layout_ == kPanels) || !pan_stale_) return;
     hip_check(launch_gather_vals(A_.nnz, pperm_.ptr, A_.val, pval_.ptr, A_.val_f32, s), "panel values");
     pan_stale_ = false;
+}
+
+// Wall time of a short CG run on `layout` (24 rounds: three graph chunks,
+// after one untimed run that captures the graph) with b = 1 and a unit Jacobi
+// diagonal; v holds 3n doubles (x, b, diagonal) with b and the diagonal = 1.
+double Operator::time_solve_layout(int layout, double* v, hipStream_t s) {
+    const int saved = solve_layout_;
+    solve_layout_ = layout;
+    drop_graph();
+    refresh_copies(s);
+    const size_t n = (size_t)A_.n;
+    (void)solve_multi(v + 2 * n, v + n, v, 23, 0.0, nullptr, s);
+    const auto t0 = std::chrono::steady_clock::now();
+    (void)solve_multi(v + 2 * n, v + n, v, 23, 0.0, nullptr, s);
+    const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    solve_layout_ = saved;
+    return t;
 }
 
 // time 5 SpMV launches on `layout` (after 2 warm-up launches); leaves that layout switched on
 float Operator::time_layout(int layout, hipStream_t s) {
     A_.pan_on = layout == kPanels;
     A_.pat_on = layout == kPatterns;
-    refresh_panels(s);
+    A_.dia_on = layout == kDiagonals;
+    refresh_copies(s);
     hip_check(hipMemsetAsync(r_.ptr, 0, sizeof(double) * (size_t)A_.n, s), "clear timing input");
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hip_check(hipEventCreate(&e0), "event");
@@ -390,18 +534,29 @@ float Operator::time_layout(int layout, hipStream_t s) {
 
 void Operator::set_layout(int layout, hipStream_t s) {
     int want = layout == kLayoutAuto ? auto_layout_ : layout;
-    if (want != kTiles && want != kPanels && want != kPatterns)
-        throw invalid("layout must be 0 (auto), 1 (tiles), 2 (panels) or 3 (column-offset patterns).");
+    if (want != kTiles && want != kPanels && want != kPatterns && want != kDiagonals)
+        throw invalid("layout must be 0 (auto), 1 (tiles), 2 (panels), 3 (column-offset patterns) or 4 "
+                      "(diagonals).");
     if (want == kPanels && !has_pan_ && !build_panels(s))
         throw invalid("the panel layout needs each row's columns in ascending order and fewer than 2^26 "
                       "(row block, column panel) segments.");
     if (want == kPatterns && !has_pat_ && !build_patterns(s))
         throw invalid("column-offset patterns need every row tile to use at most 32 distinct offsets (col - row), "
                       "rows of at most 32 entries and strictly ascending columns.");
-    const bool pan = want == kPanels, pat = want == kPatterns;
-    if (pan == A_.pan_on && pat == A_.pat_on) return;
+    if (want == kDiagonals && !has_dia_ && !build_diagonals(s))
+        throw invalid("the diagonal layout needs at most 32 distinct offsets (col - row) over the whole matrix, "
+                      "strictly ascending columns in every row and diagonals no longer than twice the stored "
+                      "entries.");
+    const bool pan = want == kPanels, pat = want == kPatterns, dia = want == kDiagonals;
+    const int solve = layout == kLayoutAuto ? auto_solve_ : want;
+    if (solve == kPanels && !has_pan_ && !build_panels(s)) throw internal("panel layout lost.");
+    if (solve == kPatterns && !has_pat_ && !build_patterns(s)) throw internal("pattern layout lost.");
+    if (solve == kDiagonals && !has_dia_ && !build_diagonals(s)) throw internal("diagonal layout lost.");
+    if (pan == A_.pan_on && pat == A_.pat_on && dia == A_.dia_on && solve == solve_layout_) return;
     A_.pan_on = pan;
     A_.pat_on = pat;
+    A_.dia_on = dia;
+    solve_layout_ = solve;
     drop_graph();
 }
 
@@ -425,7 +580,7 @@ void Operator::set_compact(bool on) {
 }
 
 void Operator::spmv(const double* x, double* y, hipStream_t s) const {
-    refresh_panels(s);
+    refresh_copies(s);
     hip_check(launch_spmv(A_, x, y, nullptr, s), "spmv launch");
 }
 
@@ -437,7 +592,7 @@ double Operator::dot(const double* a, const double* b, int n, hipStream_t s) {
 }
 
 double Operator::residual(const double* b, const double* x, double* r, hipStream_t s) {
-    refresh_panels(s);
+    refresh_copies(s);
     hip_check(launch_spmv(A_, x, r, b, s), "residual launch");
     return dot(r, r, A_.n, s);
 }
@@ -450,10 +605,10 @@ SolveResult Operator::solve(const double* diag, const double* b, double* x, uint
                             void* records, hipStream_t s) {
     if (diag == nullptr || b == nullptr || x == nullptr) throw invalid("Vector is nullptr.");
     if (max_iter == 0xFFFFFFFFu) throw invalid("maxIter too large.");
-    refresh_panels(s);
+    refresh_copies(s);
     int p = path_;
     if (p == kAuto) p = kMultiKernel;
-    if (!coop_ok_ || A_.pan_on) p = kMultiKernel;  // the persistent kernel runs the tile SpMV only
+    if (!coop_ok_ || solve_layout_ == kPanels) p = kMultiKernel;  // the persistent kernel runs the tile SpMV only
     last_path_ = p;
     if (p == kPersistent) return solve_persistent(diag, b, x, max_iter, tol, records, s);
     return solve_multi(diag, b, x, max_iter, tol, records, s);
@@ -474,14 +629,18 @@ CgParams Operator::cg_params(const double* diag, const double* b, double* x, uin
                              void* records) const {
     CgParams P;
     P.n = A_.n;
-    P.pan_on = A_.pan_on ? 1 : 0;
+    // the solve layout (which may differ from the standalone SpMV's)
+    P.pan_on = solve_layout_ == kPanels ? 1 : 0;
     P.pan = A_.pan;
-    P.pat_on = A_.pat_on ? 1 : 0;
+    P.pat_on = solve_layout_ == kPatterns ? 1 : 0;
     P.pmask = A_.pmask;
     P.poffs = A_.poffs;
+    P.dia_on = solve_layout_ == kDiagonals ? 1 : 0;
+    P.dia = A_.dia;
     P.tiles = A_.ntiles;
-    // p.Ap partials: one per panel row block, or one per row tile
-    P.ntiles = A_.pan_on ? A_.pan.nb : A_.ntiles;
+    // p.Ap partials: one per panel row block, per wavefront of the diagonal
+    // layout's workgroups, or per row tile
+    P.ntiles = P.pan_on ? A_.pan.nb : P.dia_on ? dia_cg_partials(A_.n) : A_.ntiles;
     P.nnz = (int)A_.nnz;
     P.G2 = G2_;
     P.split_alpha = P.ntiles > kMaxRedundantPartials ? 1 : 0;
@@ -549,7 +708,7 @@ SolveResult Operator::solve_multi(const double* diag, const double* b, double* x
     // graph captured for (x, records, max_rounds, values) replays for any b.
     if (graph_ok_ && (!gexec_ || gkey_.x != P.x || gkey_.records != P.records || gkey_.max_rounds != P.max_rounds ||
                       gkey_.val != P.val || gkey_.col != P.col || gkey_.pan_on != P.pan_on ||
-                      gkey_.pat_on != P.pat_on))
+                      gkey_.pat_on != P.pat_on || gkey_.dia_on != P.dia_on))
         build_graph(P);
     // Enqueue iterations in chunks; after each chunk the stop flag is copied to
     // pinned memory.  The host checks chunk c-1's flag while chunk c is queued,

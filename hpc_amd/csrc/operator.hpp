@@ -114,6 +114,7 @@ class Operator {
         if (reinterpret_cast<uintptr_t>(val) & 15u) throw invalid("value array must be 16-byte aligned.");
         A_.val = val;
         pan_stale_ = true;
+        dia_stale_ = true;
         drop_graph();
     }
     // SpMV layout: kTiles (row tiles over the CSR arrays) or kPanels (the
@@ -122,9 +123,15 @@ class Operator {
     // 15% faster on an operator whose columns do not fit the int16 stream.
     // kPatterns runs the row tiles on column-offset patterns (stencil
     // operators; bit-identical to kTiles), chosen at creation when faster.
-    enum Layout { kLayoutAuto = 0, kTiles = 1, kPanels = 2, kPatterns = 3 };
+    // kDiagonals runs on one value array per global column offset (stencils
+    // with at most 32 distinct offsets over the whole matrix).
+    enum Layout { kLayoutAuto = 0, kTiles = 1, kPanels = 2, kPatterns = 3, kDiagonals = 4 };
     void set_layout(int layout, hipStream_t s);
-    int layout() const { return A_.pan_on ? kPanels : A_.pat_on ? kPatterns : kTiles; }
+    // layout of the standalone SpMV / residual, and of the solves
+    int layout() const {
+        return A_.pan_on ? kPanels : A_.pat_on ? kPatterns : A_.dia_on ? kDiagonals : kTiles;
+    }
+    int solve_layout() const { return solve_layout_; }
     // use (or stop using) the int16 column stream when it exists
     void set_compact(bool on);
 
@@ -164,8 +171,13 @@ class Operator {
     // panel layout: build (host plan + upload), release, refresh values, time
     bool build_panels(hipStream_t s, const int32_t* rp_host = nullptr, const int32_t* col_host = nullptr);
     bool build_patterns(hipStream_t s);
+    bool build_diagonals(hipStream_t s);
+    void free_diagonals();
+    // the panel and diagonal layouts' own value copies follow A_.val before use
+    void refresh_copies(hipStream_t s) const;
     void copy_pattern_to_host(hipStream_t s, std::vector<int32_t>& rp, std::vector<int32_t>& col) const;
     float time_layout(int layout, hipStream_t s);
+    double time_solve_layout(int layout, double* v, hipStream_t s);
     void free_panels();
     void free_patterns();
     void refresh_panels(hipStream_t s) const;
@@ -203,7 +215,15 @@ class Operator {
     DevBuf<int32_t> poffs_;
     bool has_pat_ = false;
     bool pat_tried_ = false;          // the pattern plan was attempted (and failed if !has_pat_)
-    int auto_layout_ = kTiles;        // creation-time choice
+    DevBuf<int32_t> doff_;
+    DevBuf<uint32_t> dmask_;
+    DevBuf<double> dval_;             // ndia*n values of the operator's type (fp32 ones packed two per double)
+    bool has_dia_ = false;
+    bool dia_tried_ = false;
+    mutable bool dia_stale_ = false;  // diagonal values differ from A_.val
+    int auto_layout_ = kTiles;        // creation-time choice for the standalone SpMV
+    int auto_solve_ = kTiles;         // creation-time choice for the solves
+    int solve_layout_ = kTiles;
     double* host_result_ = nullptr;  // pinned
     int* host_flags_ = nullptr;      // pinned
     hipStream_t work_ = nullptr;     // graphs are captured and replayed here

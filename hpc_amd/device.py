@@ -110,11 +110,11 @@ class CsrOperator:
         check_dev(lib().xjpcg_op_set_compact(self._h, -1, ctypes.byref(act)))
         return bool(act.value)
 
-    LAYOUT_AUTO, LAYOUT_TILES, LAYOUT_PANELS, LAYOUT_PATTERNS = 0, 1, 2, 3
+    LAYOUT_AUTO, LAYOUT_TILES, LAYOUT_PANELS, LAYOUT_PATTERNS, LAYOUT_DIAGONALS = 0, 1, 2, 3, 4
 
     def set_layout(self, layout: int) -> int:
         """SpMV layout: 0 the creation-time choice, 1 row tiles, 2 panel-swept copy, 3 row tiles on column-offset
-        patterns; returns the layout in use."""
+        patterns, 4 diagonals; returns the layout in use."""
         act = ctypes.c_int()
         check_dev(lib().xjpcg_op_set_layout(self._h, int(layout), _stream(), ctypes.byref(act)))
         return act.value
@@ -123,6 +123,13 @@ class CsrOperator:
     def layout(self) -> int:
         act = ctypes.c_int()
         check_dev(lib().xjpcg_op_set_layout(self._h, -1, None, ctypes.byref(act)))
+        return act.value
+
+    @property
+    def solve_layout(self) -> int:
+        """Layout the solves run on (set_layout(L > 0) sets both; 0 restores both creation-time choices)."""
+        act = ctypes.c_int()
+        check_dev(lib().xjpcg_op_solve_layout(self._h, ctypes.byref(act)))
         return act.value
 
     PATH_AUTO, PATH_PERSISTENT, PATH_MULTI = 0, 1, 2

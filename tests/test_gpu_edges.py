@@ -1,4 +1,5 @@
-"""Edge cases of the SpMV in every layout that applies (row tiles, panel-swept copy, column-offset patterns):
+"""Edge cases of the SpMV in every layout that applies (row tiles, panel-swept copy, column-offset patterns,
+diagonals):
 a 1x1 matrix, empty rows, rows around the 2044-entry tile limit, odd entry counts (the final partial
 16-byte run), an operator with no stored entries, and the solves they allow."""
 import sys
@@ -57,7 +58,7 @@ def case(name):
     raise ValueError(name)
 
 
-LAYOUTS = {1: "tiles", 2: "panels", 3: "patterns"}
+LAYOUTS = {1: "tiles", 2: "panels", 3: "patterns", 4: "diagonals"}
 
 
 @pytest.mark.parametrize("name", ["n1", "empty_rows", "tile_limit_rows", "odd_nnz"])

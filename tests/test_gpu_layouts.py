@@ -1,6 +1,7 @@
 """SpMV layouts on the GPU.  Panel-swept copy: SpMV bit-exact against the oracle (every row, long ones
 included), solve parity, value refresh, split-diagonal fp32 operators.  Column-offset pattern tiles: SpMV
-bit-exact against the oracle and solves bit-identical to the plain row tiles."""
+bit-exact against the oracle and solves bit-identical to the plain row tiles.  Diagonals: SpMV bit-exact
+against the oracle, solves deterministic and at oracle parity."""
 import sys
 
 import numpy as np
@@ -135,7 +136,7 @@ def test_pattern_tiles_bitexact_and_identical_solves(cuda, kind):
     ref = oracle.spmv(A.rowptr, A.col, A.val, xv)
     diag, b = _t(cuda, A.diag), _t(cuda, s.b)
     sols = {}
-    for layout in (1, 3):
+    for layout in (1, 3, 4):
         assert op.set_layout(layout) == layout
         assert np.array_equal(op.spmv(x).cpu().numpy(), ref)
         r, _ = op.residual(b, x)
@@ -144,9 +145,13 @@ def test_pattern_tiles_bitexact_and_identical_solves(cuda, kind):
         sols[layout] = (xs.cpu().numpy(), info.executed)
     # identical SpMV and identical per-tile p.Ap partials: identical solves
     assert sols[1][1] == sols[3][1] and np.array_equal(sols[1][0], sols[3][0])
+    # diagonals: the same SpMV, p.Ap partials per 256 rows instead of per tile -- deterministic, oracle parity
+    xs2, info2 = op.solve(diag, b, s.max_iter, s.tol)
+    assert info2.executed == sols[4][1] and np.array_equal(xs2.cpu().numpy(), sols[4][0])
     o = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, s.b, s.max_iter, s.tol)
-    assert abs(sols[3][1] - o["executed"]) <= 1
-    assert np.linalg.norm(sols[3][0] - o["x"]) / np.linalg.norm(o["x"]) <= 1e-10
+    for layout in (3, 4):
+        assert abs(sols[layout][1] - o["executed"]) <= 1
+        assert np.linalg.norm(sols[layout][0] - o["x"]) / np.linalg.norm(o["x"]) <= 1e-10
 
 
 def test_pattern_tiles_split_fp32(cuda):
@@ -160,6 +165,8 @@ def test_pattern_tiles_split_fp32(cuda):
     y1 = op.spmv(x).clone()
     assert op.set_layout(3) == 3
     assert bool((op.spmv(x) == y1).all())
+    assert op.set_layout(4) == 4
+    assert bool((op.spmv(x) == y1).all())
 
 
 def test_pattern_layout_refused_for_graphs(cuda):
@@ -169,6 +176,8 @@ def test_pattern_layout_refused_for_graphs(cuda):
     op = device.CsrOperator(_t(cuda, A.rowptr), _t(cuda, A.col), _t(cuda, A.val), rowptr_host=A.rowptr)
     with pytest.raises(XjpcgError):
         op.set_layout(3)
+    with pytest.raises(XjpcgError):
+        op.set_layout(4)
     assert op.layout in (1, 2)
 
 
@@ -212,9 +221,11 @@ def test_pattern_tiles_more_operators(cuda, case):
         rp, col, val = _band_csr(6000, list(range(-16, 16)), list(range(-16, 16)))
     n = rp.size - 1
     op = device.CsrOperator(_t(cuda, rp), _t(cuda, col), _t(cuda, val), rowptr_host=rp)
-    assert op.set_layout(3) == 3
     xv = workloads.uniform(8, 1, n, -1, 1)
-    assert np.array_equal(op.spmv(_t(cuda, xv)).cpu().numpy(), oracle.spmv(rp, col, val, xv))
+    ref = oracle.spmv(rp, col, val, xv)
+    for layout in (3, 4):
+        assert op.set_layout(layout) == layout
+        assert np.array_equal(op.spmv(_t(cuda, xv)).cpu().numpy(), ref)
 
 
 def test_pattern_layout_refused_beyond_32_offsets(cuda):
@@ -226,5 +237,81 @@ def test_pattern_layout_refused_beyond_32_offsets(cuda):
     op = device.CsrOperator(_t(cuda, rp), _t(cuda, col), _t(cuda, val), rowptr_host=rp)
     with pytest.raises(XjpcgError):
         op.set_layout(3)
+    with pytest.raises(XjpcgError):
+        op.set_layout(4)                      # 64 offsets over the whole matrix
     xv = workloads.uniform(8, 2, rp.size - 1, -1, 1)
     assert np.array_equal(op.spmv(_t(cuda, xv)).cpu().numpy(), oracle.spmv(rp, col, val, xv))
+
+
+def test_diagonals_refused_for_tile_local_patterns(cuda):
+    """Every tile within 32 offsets but 60 over the whole matrix (the band drifts by one column every 1000
+    rows): patterns apply, diagonals are refused."""
+    from hpc_amd import device
+    from hpc_amd._lib import XjpcgError
+    n = 40000
+    rows, cols = [], []
+    for i in range(n):
+        a = 4 + i // 1000
+        c = i + np.array([-3, -1, 0, 1, 3] + list(range(a, a + 16)))
+        c = np.sort(c[(c >= 0) & (c < n)])
+        rows.append(np.full(c.size, i))
+        cols.append(c)
+    rows, cols = np.concatenate(rows), np.concatenate(cols)
+    rp = np.zeros(n + 1, np.int32)
+    np.cumsum(np.bincount(rows, minlength=n), out=rp[1:])
+    col = cols.astype(np.int32)
+    val = np.random.default_rng(3).standard_normal(col.size)
+    op = device.CsrOperator(_t(cuda, rp), _t(cuda, col), _t(cuda, val), rowptr_host=rp)
+    assert op.set_layout(3) == 3
+    with pytest.raises(XjpcgError):
+        op.set_layout(4)
+    assert op.layout == 3
+    xv = workloads.uniform(8, 4, n, -1, 1)
+    assert np.array_equal(op.spmv(_t(cuda, xv)).cpu().numpy(), oracle.spmv(rp, col, val, xv))
+
+
+def test_diagonals_value_refresh_and_boundary_rows(cuda):
+    """Diagonals keep their own value copy: set_values (and values changed in place) refresh it before the
+    next SpMV; boundary rows with missing diagonals and a row with no entries at all."""
+    import torch
+    from hpc_amd import device
+    A = workloads.stencil(workloads.STENCIL_3D7, 21, 13, 7, [0.5, 1.5, 2.5])
+    rp, col = A.rowptr.copy(), A.col.copy()
+    val = A.val.copy()
+    # empty the last row
+    k0, k1 = rp[-2], rp[-1]
+    col = np.delete(col, np.arange(k0, k1))
+    val = np.delete(val, np.arange(k0, k1))
+    rp[-1] = k0
+    n = rp.size - 1
+    v = _t(cuda, val)
+    op = device.CsrOperator(_t(cuda, rp), _t(cuda, col), v, rowptr_host=rp)
+    assert op.set_layout(4) == 4
+    xv = workloads.uniform(8, 3, n, -1, 1)
+    x = _t(cuda, xv)
+    assert np.array_equal(op.spmv(x).cpu().numpy(), oracle.spmv(rp, col, val, xv))
+    v.mul_(-2.5)                              # in place, then announced
+    op.set_values(v)
+    assert np.array_equal(op.spmv(x).cpu().numpy(), oracle.spmv(rp, col, -2.5 * val, xv))
+    w = torch.from_numpy(np.ascontiguousarray(val * 3.0)).to(cuda)
+    op.set_values(w)
+    assert np.array_equal(op.spmv(x).cpu().numpy(), oracle.spmv(rp, col, 3.0 * val, xv))
+
+
+def test_solve_layout_chosen_separately(cuda):
+    """The solves keep their own creation-time layout; set_layout(L > 0) sets both, 0 restores both."""
+    from hpc_amd import device
+    s = workloads.c2_stencil27(40)
+    A = s.A
+    op = device.CsrOperator(_t(cuda, A.rowptr), _t(cuda, A.col), _t(cuda, A.val), rowptr_host=A.rowptr)
+    spmv0, solve0 = op.layout, op.solve_layout
+    _say(f"creation-time layouts: spmv {spmv0} solve {solve0}")
+    assert spmv0 in (1, 3, 4) and solve0 in (1, 3, 4)
+    diag, b = _t(cuda, A.diag), _t(cuda, s.b)
+    x_auto, info_auto = op.solve(diag, b, s.max_iter, s.tol)
+    assert op.set_layout(solve0) == solve0 and op.solve_layout == solve0
+    x_same, info_same = op.solve(diag, b, s.max_iter, s.tol)
+    assert info_same.executed == info_auto.executed and bool((x_same == x_auto).all())
+    for lay in (1, 3, 4):
+        assert op.set_layout(lay) == lay and op.solve_layout == lay
+    assert op.set_layout(0) == spmv0 and op.solve_layout == solve0
