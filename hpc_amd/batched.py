@@ -108,10 +108,28 @@ class Batch:
     def handle(self):
         return self._h
 
+    LAYOUT_AUTO, LAYOUT_CSR, LAYOUT_DIAGONALS = 0, 1, 2
+
     def info(self):
         v = [ctypes.c_int32() for _ in range(5)]
         _check(lib().xjpcg_batch_info(self._h, *[ctypes.byref(a) for a in v]))
-        return dict(zip(["rows", "nnz", "tiles", "tile_rows", "block"], [a.value for a in v]))
+        d = dict(zip(["rows", "nnz", "tiles", "tile_rows", "block"], [a.value for a in v]))
+        d["layout"] = {1: "csr", 2: "diagonals"}[self.layout]
+        return d
+
+    def set_layout(self, layout: int) -> int:
+        """SpMM storage: 0 the creation-time / autotuned choice, 1 CSR, 2 per-system diagonals; returns the one
+        in use."""
+        act = ctypes.c_int32()
+        _check(lib().xjpcg_batch_set_layout(self._h, int(layout), ctypes.byref(act),
+                                            ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        return act.value
+
+    @property
+    def layout(self) -> int:
+        act = ctypes.c_int32()
+        _check(lib().xjpcg_batch_set_layout(self._h, -1, ctypes.byref(act), None))
+        return act.value
 
     def configure(self, tile_rows: int = 0, block: int = 0, tune: bool = False):
         _check(lib().xjpcg_batch_set_config(self._h, int(tile_rows), int(block), 1 if tune else 0))

@@ -176,12 +176,55 @@ __device__ __forceinline__ void brow(const BatchParams& P, const VT* __restrict_
     }
 }
 
+// The same row on the per-system diagonal layout: the row's zero-padded
+// values, the R/Q-wide x segments of every offset (columns clamped into the
+// system for offsets the row lacks) and its mask load together, then the
+// entries are added in ascending offset order -- the CSR order of the row,
+// so the sums are those of brow.
 template <typename VT, int R>
+__device__ __forceinline__ void brow_dia(const BatchParams& P, const BDiaSys& Y, const VT* __restrict__ dval,
+                                         const double* __restrict__ xin, int i, int q0,
+                                         double (&acc)[SpmmLanes<R>::Q]) {
+    constexpr int Q = SpmmLanes<R>::Q;
+    const uint32_t m = P.dmask[i];
+    if (P.dsplit) {
+        double xi[Q];
+        load_q<Q>(xin + (size_t)i * R + q0, xi);
+        const double d = P.dsplit[i];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) acc[q] = d * xi[q];
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) acc[q] = 0.0;
+    }
+    const int ns = Y.r1 - Y.r0, li = i - Y.r0;
+    double v[kBDiaMax];
+    double g[kBDiaMax][Q];
+#pragma unroll
+    for (int d = 0; d < kBDiaMax; ++d) {
+        if (d < Y.nd) {
+            v[d] = (double)ld_stream(dval + Y.base + (size_t)d * ns + li);
+            int c = i + Y.off[d];
+            c = c < Y.r0 ? Y.r0 : (c >= Y.r1 ? Y.r1 - 1 : c);
+            load_q<Q>(xin + (size_t)c * R + q0, g[d]);
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < kBDiaMax; ++d)
+        if (d < Y.nd && ((m >> d) & 1u)) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) acc[q] += v[d] * g[d][q];
+        }
+}
+
+template <typename VT, int R, bool DIA>
 __global__ __launch_bounds__(kMaxBlock) void bspmm_dot(BatchParams P) {
     constexpr int Q = SpmmLanes<R>::Q, L = SpmmLanes<R>::L;
     __shared__ double red[kMaxBlock * Q];
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int s = P.tile_sys[t];
+    BDiaSys Y;
+    if (DIA) Y = P.dsys[s];
     if (!P.sys_active[s]) return;
     const int l = threadIdx.x % L, g = threadIdx.x / L, G = blockDim.x / L;
     const int q0 = l * Q;
@@ -193,7 +236,10 @@ __global__ __launch_bounds__(kMaxBlock) void bspmm_dot(BatchParams P) {
     for (int q = 0; q < Q; ++q) dl[q] = 0.0;
     for (int i = r0 + g; i < r1; i += G) {
         double acc[Q];
-        brow<VT, R>(P, val, p, i, q0, acc);
+        if (DIA)
+            brow_dia<VT, R>(P, Y, static_cast<const VT*>(P.dval), p, i, q0, acc);
+        else
+            brow<VT, R>(P, val, p, i, q0, acc);
         double pi[Q];
         load_q<Q>(p + (size_t)i * R + q0, pi);
 #pragma unroll
@@ -215,18 +261,23 @@ __global__ __launch_bounds__(kMaxBlock) void bspmm_dot(BatchParams P) {
 }
 
 // plain batched SpMM y = A x (or y = b - A x) for the autotuner and the refinement residual
-template <typename VT, int R>
+template <typename VT, int R, bool DIA>
 __global__ __launch_bounds__(kMaxBlock) void bspmm(BatchParams P, const double* __restrict__ xin,
                                                    double* __restrict__ y, const double* __restrict__ b) {
     constexpr int Q = SpmmLanes<R>::Q, L = SpmmLanes<R>::L;
     const int t = xcd_remap(blockIdx.x, gridDim.x);
+    BDiaSys Y;
+    if (DIA) Y = P.dsys[P.tile_sys[t]];
     const int l = threadIdx.x % L, g = threadIdx.x / L, G = blockDim.x / L;
     const int q0 = l * Q;
     const int r0 = P.tile_row[t], r1 = P.tile_row[t + 1];
     const VT* __restrict__ val = static_cast<const VT*>(P.val);
     for (int i = r0 + g; i < r1; i += G) {
         double acc[Q];
-        brow<VT, R>(P, val, xin, i, q0, acc);
+        if (DIA)
+            brow_dia<VT, R>(P, Y, static_cast<const VT*>(P.dval), xin, i, q0, acc);
+        else
+            brow<VT, R>(P, val, xin, i, q0, acc);
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const size_t e = (size_t)i * R + q0 + q;
@@ -399,10 +450,17 @@ __global__ void baxpy(size_t m, const double* __restrict__ d, double* __restrict
 
 template <int R>
 static void iteration_r(const BatchParams& P, bool f32, int block, hipStream_t s) {
-    if (f32)
-        bspmm_dot<float, R><<<P.T, block, 0, s>>>(P);
-    else
-        bspmm_dot<double, R><<<P.T, block, 0, s>>>(P);
+    if (P.dia_on) {
+        if (f32)
+            bspmm_dot<float, R, true><<<P.T, block, 0, s>>>(P);
+        else
+            bspmm_dot<double, R, true><<<P.T, block, 0, s>>>(P);
+    } else {
+        if (f32)
+            bspmm_dot<float, R, false><<<P.T, block, 0, s>>>(P);
+        else
+            bspmm_dot<double, R, false><<<P.T, block, 0, s>>>(P);
+    }
     bsys_alpha<R><<<P.S, kMaxBlock, 0, s>>>(P);
     bupdate_r<R><<<P.T, block, 0, s>>>(P);
     bsys_beta<R><<<P.S, kMaxBlock, 0, s>>>(P);
@@ -418,10 +476,17 @@ static void setup_r(const BatchParams& P, int block, hipStream_t s) {
 template <int R>
 static void spmm_r(const BatchParams& P, bool f32, int block, const double* x, double* y, const double* b,
                    hipStream_t s) {
-    if (f32)
-        bspmm<float, R><<<P.T, block, 0, s>>>(P, x, y, b);
-    else
-        bspmm<double, R><<<P.T, block, 0, s>>>(P, x, y, b);
+    if (P.dia_on) {
+        if (f32)
+            bspmm<float, R, true><<<P.T, block, 0, s>>>(P, x, y, b);
+        else
+            bspmm<double, R, true><<<P.T, block, 0, s>>>(P, x, y, b);
+    } else {
+        if (f32)
+            bspmm<float, R, false><<<P.T, block, 0, s>>>(P, x, y, b);
+        else
+            bspmm<double, R, false><<<P.T, block, 0, s>>>(P, x, y, b);
+    }
 }
 
 template <int R>
@@ -478,6 +543,93 @@ hipError_t batch_launch_finish(int nq, const double* dot, const double* rr, cons
     int g = (nq + 255) / 256;
     if (g < 1) g = 1;
     bfinish<<<g, 256, 0, s>>>(nq, dot, rr, iters, out_iters, out_res);
+    return hipGetLastError();
+}
+
+// Distinct column offsets of one tile (one workgroup): an LDS hash of 16
+// slots written out as is (the host forms each system's union).
+__global__ __launch_bounds__(256) void btile_offsets(BatchParams P, int* __restrict__ toffs, int* __restrict__ fail) {
+    constexpr int kSlots = 16;
+    constexpr int kEmpty = INT32_MIN;
+    __shared__ int tab[kSlots];
+    __shared__ int cnt, bad;
+    const int t = blockIdx.x;
+    if (threadIdx.x < kSlots) tab[threadIdx.x] = kEmpty;
+    if (threadIdx.x == 0) {
+        cnt = 0;
+        bad = 0;
+    }
+    __syncthreads();
+    const int s = P.tile_sys[t];
+    const int lo = P.sys_row[s], hi = P.sys_row[s + 1];
+    for (int i = P.tile_row[t] + (int)threadIdx.x; i < P.tile_row[t + 1]; i += blockDim.x) {
+        int prev = -1;
+        for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k) {
+            const int c = P.col[k];
+            if (c < lo || c >= hi || c <= prev) {
+                atomicOr(&bad, 1);
+                break;
+            }
+            prev = c;
+            const int d = c - i;
+            int h = (int)(((uint32_t)d * 2654435761u) >> 28);
+            for (int probe = 0; probe < kSlots; ++probe) {
+                const int old = atomicCAS(&tab[h], kEmpty, d);
+                if (old == kEmpty) {
+                    if (atomicAdd(&cnt, 1) >= kBDiaMax) atomicOr(&bad, 1);
+                    break;
+                }
+                if (old == d) break;
+                h = (h + 1) & (kSlots - 1);
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < kSlots) toffs[t * kSlots + threadIdx.x] = tab[threadIdx.x];
+    if (threadIdx.x == 0 && bad) atomicOr(fail, 1);
+}
+
+template <typename VT>
+__global__ __launch_bounds__(256) void bfill_dia(BatchParams P, VT* __restrict__ dval, uint8_t* __restrict__ dmask,
+                                                 int* __restrict__ fail) {
+    const int t = blockIdx.x;
+    const BDiaSys& Y = P.dsys[P.tile_sys[t]];  // read from memory: the offset scan indexes it at run time
+    const VT* __restrict__ val = static_cast<const VT*>(P.val);
+    const int ns = Y.r1 - Y.r0;
+    for (int i = P.tile_row[t] + (int)threadIdx.x; i < P.tile_row[t + 1]; i += blockDim.x) {
+        uint32_t m = 0;
+        int prev = -1;
+        bool bad = false;
+        for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k) {
+            const int off = P.col[k] - i;
+            int d = 0;
+            while (d < Y.nd && Y.off[d] != off) ++d;
+            if (d >= Y.nd || d <= prev) {
+                bad = true;
+                break;
+            }
+            prev = d;
+            m |= 1u << d;
+            dval[Y.base + (size_t)d * ns + (i - Y.r0)] = val[k];
+        }
+        dmask[i] = (uint8_t)m;
+        if (bad) atomicOr(fail, 1);
+    }
+}
+
+hipError_t batch_launch_tile_offsets(const BatchParams& P, int* toffs, int* fail, hipStream_t s) {
+    if (P.T <= 0) return hipSuccess;
+    btile_offsets<<<P.T, 256, 0, s>>>(P, toffs, fail);
+    return hipGetLastError();
+}
+
+hipError_t batch_launch_fill_dia(const BatchParams& P, bool f32, void* dval, uint8_t* dmask, int* fail,
+                                 hipStream_t s) {
+    if (P.T <= 0) return hipSuccess;
+    if (f32)
+        bfill_dia<float><<<P.T, 256, 0, s>>>(P, static_cast<float*>(dval), dmask, fail);
+    else
+        bfill_dia<double><<<P.T, 256, 0, s>>>(P, static_cast<double*>(dval), dmask, fail);
     return hipGetLastError();
 }
 

@@ -7,6 +7,17 @@
 
 namespace xj {
 
+// Diagonal layout of one system: its rows [r0, r1), nd <= kBDiaMax column
+// offsets (ascending), and where its nd value arrays of r1 - r0 entries start.
+constexpr int kBDiaMax = 8;
+struct BDiaSys {
+    int off[kBDiaMax];
+    int r0, r1, nd, pad;
+    long long base;
+    long long pad2;
+};
+static_assert(sizeof(BDiaSys) == 64, "system record layout");
+
 struct BatchParams {
     int N = 0, S = 0, T = 0;             // rows, systems, tiles
     const int* rowptr = nullptr;         // [N+1]
@@ -26,6 +37,11 @@ struct BatchParams {
     int *active = nullptr, *upd = nullptr, *iters = nullptr, *it = nullptr;  // [S][R]
     int *sys_active = nullptr, *sys_upd = nullptr;                   // [S]
     int* any_active = nullptr;           // [1]
+    const int* sys_row = nullptr;        // [S+1] first row of each system
+    int dia_on = 0;                      // 1: the SpMM runs on the per-system diagonal layout
+    const BDiaSys* dsys = nullptr;       // [S]
+    const void* dval = nullptr;          // per system nd*(r1-r0) values of the value type, zero where absent
+    const uint8_t* dmask = nullptr;      // [N]: bit d set when the row has an entry at its system's off[d]
     double tol = 0.0;
     uint32_t max_rounds = 0;
 };
@@ -40,5 +56,12 @@ hipError_t batch_launch_mask(const BatchParams& P, int R, int block, const int* 
 hipError_t batch_launch_finish(int nq, const double* dot, const double* rr, const int* iters, int* out_iters,
                                double* out_res, hipStream_t s);
 hipError_t batch_launch_axpy(size_t m, const double* d, double* x, hipStream_t s);
+// per tile: its distinct column offsets in 16 slots (INT32_MIN = unused); *fail when
+// a tile has more than kBDiaMax offsets, a column lies outside its system or a
+// row's columns do not strictly ascend
+hipError_t batch_launch_tile_offsets(const BatchParams& P, int* toffs, int* fail, hipStream_t s);
+// diagonal values and masks (dval zeroed beforehand); *fail when an entry's
+// offset is not its system's or a row's offsets do not ascend
+hipError_t batch_launch_fill_dia(const BatchParams& P, bool f32, void* dval, uint8_t* dmask, int* fail, hipStream_t s);
 
 }  // namespace xj

@@ -61,7 +61,21 @@ class Batch {
         sys_active_.ensure(S);
         sys_upd_.ensure(S);
         any_.ensure(1);
+        sysrow_.ensure((size_t)S + 1);
+        hip_check(hipMemcpyAsync(sysrow_.ptr, h_sys_row, sizeof(int32_t) * ((size_t)S + 1), hipMemcpyHostToDevice,
+                                 work_),
+                  "system rows");
         configure(default_tile_rows(), 256, user);
+        // per-system diagonal layout when every system has at most kBDiaMax
+        // offsets; kept when its SpMM is the faster one
+        if (nnz_ > 0 && build_dia()) {
+            dia_on_ = false;
+            const float t_csr = time_spmm();
+            dia_on_ = true;
+            const float t_dia = time_spmm();
+            dia_on_ = t_dia < t_csr;
+            dia_auto_ = dia_on_;
+        }
     }
 
     ~Batch() {
@@ -90,6 +104,7 @@ class Batch {
             st.push_back((int32_t)ts.size());
         }
         const int T = (int)ts.size();
+        sys_tile_h_ = st;
         sync_user(user);
         tile_row_.ensure(tr.size());
         tile_sys_.ensure(std::max<size_t>(1, ts.size()));
@@ -107,43 +122,72 @@ class Batch {
         drop_graph();
     }
 
-    // The autotuner: time the batched SpMM for each candidate (tile rows, block)
-    // and keep the fastest.
-    void autotune(hipStream_t user) {
-        static const int tiles[] = {32, 64, 128, 256, 512};
-        static const int blocks[] = {128, 256, 512};
-        sync_user(user);
+    // Mean time of 5 batched SpMM launches (after 2 untimed ones) in the
+    // current layout and tile configuration, on the work stream.
+    float time_spmm() {
         double* xs = p_.ptr;
         double* ys = Ap_.ptr;
         hip_check(hipMemsetAsync(xs, 0, (size_t)N_ * R_ * sizeof(double), work_), "tune input");
         hipEvent_t e0, e1;
         hip_check(hipEventCreate(&e0), "event");
         hip_check(hipEventCreate(&e1), "event");
+        BatchParams P = params(nullptr, nullptr, nullptr, 0.0, 1);
+        for (int w = 0; w < 2; ++w) hip_check(batch_launch_spmm(P, R_, f32_, block_, xs, ys, nullptr, work_), "tune");
+        hip_check(hipEventRecord(e0, work_), "event");
+        for (int w = 0; w < 5; ++w) hip_check(batch_launch_spmm(P, R_, f32_, block_, xs, ys, nullptr, work_), "tune");
+        hip_check(hipEventRecord(e1, work_), "event");
+        hip_check(hipEventSynchronize(e1), "event");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "event");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return ms / 5.0f;
+    }
+
+    // The autotuner: time the batched SpMM for each candidate (layout, tile
+    // rows, block) and keep the fastest.
+    void autotune(hipStream_t user) {
+        static const int tiles[] = {32, 64, 128, 256, 512};
+        static const int blocks[] = {128, 256, 512};
+        sync_user(user);
         float best = 1e30f;
         int bt = tile_rows_, bb = block_;
-        for (int tr : tiles) {
-            for (int bl : blocks) {
-                if (bl < R_ || bl / R_ > tr) continue;  // at most one row per lane group pass beyond the tile
-                configure(tr, bl, work_);
-                BatchParams P = params(nullptr, nullptr, nullptr, 0.0, 1);
-                for (int w = 0; w < 2; ++w) hip_check(batch_launch_spmm(P, R_, f32_, block_, xs, ys, nullptr, work_), "tune");
-                hip_check(hipEventRecord(e0, work_), "event");
-                for (int w = 0; w < 5; ++w) hip_check(batch_launch_spmm(P, R_, f32_, block_, xs, ys, nullptr, work_), "tune");
-                hip_check(hipEventRecord(e1, work_), "event");
-                hip_check(hipEventSynchronize(e1), "event");
-                float ms = 0.f;
-                hip_check(hipEventElapsedTime(&ms, e0, e1), "event");
-                if (ms < best) {
-                    best = ms;
-                    bt = tr;
-                    bb = bl;
+        bool bd = dia_on_;
+        for (int lay = 0; lay < (has_dia_ ? 2 : 1); ++lay) {
+            dia_on_ = lay == 1;
+            for (int tr : tiles) {
+                for (int bl : blocks) {
+                    if (bl < R_ || bl / R_ > tr) continue;  // at most one row per lane group pass beyond the tile
+                    configure(tr, bl, work_);
+                    const float ms = time_spmm();
+                    if (ms < best) {
+                        best = ms;
+                        bt = tr;
+                        bb = bl;
+                        bd = dia_on_;
+                    }
                 }
             }
         }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        dia_on_ = bd;
+        dia_auto_ = bd;
         configure(bt, bb, work_);
     }
+
+    // 0: keep the creation-time choice, 1: CSR, 2: per-system diagonals
+    void set_layout(int layout, hipStream_t user) {
+        if (layout < 0 || layout > 2) throw invalid("layout must be 0 (auto), 1 (CSR) or 2 (diagonals).");
+        if (layout == 2 && !has_dia_)
+            throw invalid("the diagonal layout needs at most 8 distinct offsets (col - row) per system, columns "
+                          "inside the row's system and strictly ascending in every row.");
+        sync_user(user);
+        const bool want = layout == 0 ? dia_auto_ : layout == 2;
+        if (want != dia_on_) {
+            dia_on_ = want;
+            drop_graph();
+        }
+    }
+    int layout() const { return dia_on_ ? 2 : 1; }
 
     int64_t solve(const double* diag, const double* b, double* x, uint32_t max_iter, double tol, int32_t* d_iters,
                   double* d_res, hipStream_t user) {
@@ -252,9 +296,89 @@ class Batch {
         P.sys_active = sys_active_.ptr;
         P.sys_upd = sys_upd_.ptr;
         P.any_active = any_.ptr;
+        P.sys_row = sysrow_.ptr;
+        P.dia_on = dia_on_ ? 1 : 0;
+        P.dsys = dsys_.ptr;
+        P.dval = dval_.ptr;
+        P.dmask = dmask_.ptr;
         P.tol = tol;
         P.max_rounds = rounds;
         return P;
+    }
+
+    // Per-system offsets from a per-tile plan on the GPU (union per system on
+    // the host), then the value arrays and masks on the GPU.  Refused (false)
+    // beyond kBDiaMax offsets in a system, for columns outside the row's system
+    // or not strictly ascending, or value arrays beyond twice the stored entries.
+    bool build_dia() {
+        BatchParams P = params(nullptr, nullptr, nullptr, 0.0, 1);
+        std::vector<int32_t> toffs((size_t)T_ * 16);
+        {
+            DevBuf<int32_t> dto;
+            DevBuf<int> fail;
+            dto.ensure(toffs.size());
+            fail.ensure(1);
+            hip_check(hipMemsetAsync(fail.ptr, 0, sizeof(int), work_), "reset flag");
+            hip_check(batch_launch_tile_offsets(P, dto.ptr, fail.ptr, work_), "tile offsets");
+            int f = 1;
+            hip_check(hipMemcpyAsync(toffs.data(), dto.ptr, sizeof(int32_t) * toffs.size(), hipMemcpyDeviceToHost,
+                                     work_),
+                      "tile offsets");
+            hip_check(hipMemcpyAsync(&f, fail.ptr, sizeof(int), hipMemcpyDeviceToHost, work_), "tile offsets");
+            hip_check(hipStreamSynchronize(work_), "tile offsets");
+            if (f) return false;
+        }
+        std::vector<BDiaSys> sys((size_t)S_);
+        int64_t total = 0;
+        for (int s = 0; s < S_; ++s) {
+            std::vector<int32_t> u;
+            for (int t = sys_tile_h_[s]; t < sys_tile_h_[s + 1]; ++t)
+                for (int j = 0; j < 16; ++j)
+                    if (toffs[(size_t)t * 16 + j] != INT32_MIN) u.push_back(toffs[(size_t)t * 16 + j]);
+            std::sort(u.begin(), u.end());
+            u.erase(std::unique(u.begin(), u.end()), u.end());
+            if ((int)u.size() > kBDiaMax) return false;
+            BDiaSys& Y = sys[s];
+            std::memset(&Y, 0, sizeof(Y));
+            for (size_t d = 0; d < u.size(); ++d) Y.off[d] = u[d];
+            Y.r0 = sys_row_[s];
+            Y.r1 = sys_row_[s + 1];
+            Y.nd = (int)u.size();
+            Y.base = total;
+            total += (int64_t)Y.nd * (Y.r1 - Y.r0);
+        }
+        if (total > 2 * (int64_t)nnz_ + N_) return false;
+        const size_t vbytes = (size_t)total * (f32_ ? 4 : 8);
+        try {
+            dsys_.ensure(sys.size());
+            dval_.ensure(std::max<size_t>(1, (vbytes + 7) / 8));
+            dmask_.ensure((size_t)N_);
+        } catch (const Error& e) {
+            if (e.status != XJPCG_STATUS_ALLOC_FAILED) throw;
+            dsys_.release();
+            dval_.release();
+            dmask_.release();
+            return false;
+        }
+        hip_check(hipMemcpyAsync(dsys_.ptr, sys.data(), sizeof(BDiaSys) * sys.size(), hipMemcpyHostToDevice, work_),
+                  "system diagonals");
+        hip_check(hipMemsetAsync(dval_.ptr, 0, vbytes, work_), "clear diagonals");
+        DevBuf<int> fail;
+        fail.ensure(1);
+        hip_check(hipMemsetAsync(fail.ptr, 0, sizeof(int), work_), "reset flag");
+        P = params(nullptr, nullptr, nullptr, 0.0, 1);
+        hip_check(batch_launch_fill_dia(P, f32_, dval_.ptr, dmask_.ptr, fail.ptr, work_), "fill diagonals");
+        int f = 1;
+        hip_check(hipMemcpyAsync(&f, fail.ptr, sizeof(int), hipMemcpyDeviceToHost, work_), "fill diagonals");
+        hip_check(hipStreamSynchronize(work_), "fill diagonals");  // host tables end with this scope
+        if (f) {
+            dsys_.release();
+            dval_.release();
+            dmask_.release();
+            return false;
+        }
+        has_dia_ = true;
+        return true;
     }
 
     void build_graph(const BatchParams& P) {
@@ -306,7 +430,12 @@ class Batch {
     hipGraphExec_t gexec_ = nullptr;
     double* g_x_ = nullptr;
     uint32_t g_rounds_ = 0;
-    DevBuf<int32_t> tile_row_, tile_sys_, sys_tile_;
+    DevBuf<int32_t> tile_row_, tile_sys_, sys_tile_, sysrow_;
+    std::vector<int32_t> sys_tile_h_;
+    DevBuf<BDiaSys> dsys_;
+    DevBuf<double> dval_;            // per-system diagonals of the value type (fp32 ones packed two per double)
+    DevBuf<uint8_t> dmask_;
+    bool has_dia_ = false, dia_on_ = false, dia_auto_ = false;
     DevBuf<double> r_, p_, Ap_, J_, partA_, partB_, partC_;
     DevBuf<double> dot_, tols_, rz_, rr_, alpha_, beta_;
     DevBuf<int> active_, upd_, iters_, it_, sys_active_, sys_upd_, any_;
@@ -384,6 +513,15 @@ extern "C" XJPCG_Status_t xjpcg_batch_set_config(xjpcg_batch_t* batch, int32_t t
             B.autotune(nullptr);
         else
             B.configure(tile_rows, block, nullptr);
+    });
+}
+
+extern "C" XJPCG_Status_t xjpcg_batch_set_layout(xjpcg_batch_t* batch, int32_t layout, int32_t* h_active,
+                                                 void* stream) {
+    return bguard([&] {
+        xj::Batch& B = impl_of(batch);
+        if (layout >= 0) B.set_layout(layout, static_cast<hipStream_t>(stream));
+        if (h_active) *h_active = B.layout();
     });
 }
 

@@ -48,6 +48,15 @@ XJPCG_Status_t xjpcg_batch_info(const xjpcg_batch_t* batch, int32_t* nrows, int3
  * configurations on this batch's SpMM and keeps the fastest (the autotuner),
  * ignoring the other two arguments. */
 XJPCG_Status_t xjpcg_batch_set_config(xjpcg_batch_t* batch, int32_t tile_rows, int32_t block, int32_t tune);
+/* Storage of the SpMM: 1 = the block-diagonal CSR; 2 = per-system diagonals
+ * (each system's at most 8 distinct offsets col - row, one value array per
+ * offset with zeros where a row lacks it, a presence byte per row: every
+ * load of a row goes out at once; the sums are those of the CSR, in column
+ * order).  At creation the batch builds the diagonals when every system
+ * allows them and keeps the faster SpMM; the autotuner tunes the layout with
+ * the tile configuration.  layout = 0 restores that choice, < 0 only queries;
+ * *h_active (may be NULL) receives the layout in use. */
+XJPCG_Status_t xjpcg_batch_set_layout(xjpcg_batch_t* batch, int32_t layout, int32_t* h_active, void* stream);
 
 /* Solve every (system, rhs) pair from x0 = 0.  d_b, d_x: [N][R].
  * d_iters (int32 [S][R], may be NULL) receives the iterations each pair ran,

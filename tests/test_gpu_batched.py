@@ -115,3 +115,49 @@ def test_batched_mixed_refined(cuda):
             bj = rhs[s][:, j]
             true = np.linalg.norm(bj - M @ xh[r0:r1, j]) / np.linalg.norm(bj)
             assert true < tol * 1.01, (s, j, true)
+
+
+@pytest.mark.parametrize("split", [False, True], ids=["fp64", "fp32-split"])
+def test_batched_diagonals_match_csr(cuda, split):
+    """Per-system diagonals (at most 8 offsets per system): the SpMM sums each row in CSR order, so solves on
+    both layouts are identical, pair for pair; a system of more than 8 offsets refuses the layout."""
+    from hpc_amd import batched
+    from hpc_amd._lib import XjpcgError
+    mats, rhs = small_batch(count=5, nrhs=4, seed=11)
+    asm = batched.assemble(mats, split_fp32=split)
+    B = batched.Batch.from_host(asm, 4, cuda)
+    _say(f"diagonals: creation-time layout {B.layout} info={B.info()}")
+    b = _dev(cuda, np.concatenate(rhs))
+    diag = _dev(cuda, asm["diag"])
+    out = {}
+    for lay in (1, 2):
+        assert B.set_layout(lay) == lay
+        x, info = B.solve(diag, b, 5000, 1e-8)
+        out[lay] = (x.cpu().numpy(), info.iterations.cpu().numpy())
+        if split:  # refinement: the residual SpMM runs on the outer batch's layout
+            outer = batched.Batch.from_host(asm, 4, cuda)
+            assert outer.set_layout(lay) == lay
+            xm, _, steps = batched.solve_mixed(outer, B, diag, b, 1e-8, inner_tol=0.0)
+            out[lay] += (xm.cpu().numpy(), steps)
+    assert np.array_equal(out[1][1], out[2][1])
+    assert np.array_equal(out[1][0], out[2][0])
+    if split:
+        assert out[1][3] == out[2][3] and np.array_equal(out[1][2], out[2][2])
+    # an extra coupling beyond the system's stencil: 9 offsets in one system
+    A0 = mats[0]
+    rows = np.repeat(np.arange(A0.n), np.diff(A0.rowptr))
+    cols = A0.col.astype(np.int64)
+    vals = A0.val.copy()
+    extra_r = np.array([0, A0.n - 1, 5, 17])
+    extra_c = np.array([A0.n // 2, A0.n // 3, 5 + 37, 17 + 101])
+    rows = np.concatenate([rows, extra_r, extra_c])
+    cols = np.concatenate([cols, extra_c, extra_r])
+    vals = np.concatenate([vals, np.full(8, -1e-3)])
+    o = np.lexsort((cols, rows))
+    rp = np.zeros(A0.n + 1, np.int64)
+    np.cumsum(np.bincount(rows, minlength=A0.n), out=rp[1:])
+    A1 = workloads.CsrMatrix(rp.astype(np.int32), cols[o].astype(np.int32), vals[o], A0.diag)
+    B2 = batched.Batch.from_host(batched.assemble([A1] + mats[1:]), 4, cuda)
+    assert B2.layout == 1
+    with pytest.raises(XjpcgError):
+        B2.set_layout(2)
