@@ -298,14 +298,18 @@ template <int R>
 __global__ __launch_bounds__(kMaxBlock) void bsys_alpha(BatchParams P) {
     __shared__ double red[kMaxBlock];
     const int s = blockIdx.x;
-    if (s == 0 && threadIdx.x == 0) *P.any_active = 0;  // bsys_beta sets it again for systems still running
-    if (!P.sys_active[s]) return;
-    const int j = threadIdx.x % R, g = threadIdx.x / R;
-    const double pap = system_sum<R>(P.partA, P.sys_tile[s], P.sys_tile[s + 1], red);
-    if (g == 0) {
-        const int q = s * R + j;
-        P.alpha[q] = P.active[q] ? P.rz[q] / pap : 0.0;
+    if (P.sys_active[s]) {
+        const int j = threadIdx.x % R, g = threadIdx.x / R;
+        const double pap = system_sum<R>(P.partA, P.sys_tile[s], P.sys_tile[s + 1], red);
+        if (g == 0) {
+            const int q = s * R + j;
+            P.alpha[q] = P.active[q] ? P.rz[q] / pap : 0.0;
+        }
     }
+    // cleared after this workgroup's loads (a store ahead of them would turn
+    // their scalar loads into vector loads); bsys_beta sets it again for
+    // systems still running
+    if (s == 0 && threadIdx.x == 0) *P.any_active = 0;
 }
 
 template <int R>

@@ -249,7 +249,6 @@ __global__ __launch_bounds__(kBlock) void cg_spmv(CgParams P) {
         }
     }
     if (stop) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->k = st->k + 1;  // no other workgroup of this launch reads k
 #pragma unroll
     for (int u = 0; u < kRowTiles; ++u) {
         if (T[u].t < 0) break;
@@ -264,6 +263,9 @@ __global__ __launch_bounds__(kBlock) void cg_spmv(CgParams P) {
         pap = block_sum(pap, red);
         if (threadIdx.x == 0) P.partA[T[u].t] = pap;
     }
+    // k advances after the last load: a store ahead of the loads could alias
+    // the tables and would turn their scalar loads into vector loads
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->k = st->k + 1;  // no other workgroup of this launch reads k
 }
 
 // CG-loop SpMV on column-offset pattern tiles: logical workgroup w runs tiles
@@ -292,7 +294,6 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_pattern(CgParams P) {
         }
     }
     if (stop) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->k = st->k + 1;  // no other workgroup of this launch reads k
 #pragma unroll
     for (int u = 0; u < kPatternTiles; ++u) {
         if (T[u].t < 0) break;
@@ -307,6 +308,9 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_pattern(CgParams P) {
         pap = block_sum(pap, red);
         if (threadIdx.x == 0) P.partA[T[u].t] = pap;
     }
+    // k advances after the last load: a store ahead of the loads could alias
+    // the tables and would turn their scalar loads into vector loads
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->k = st->k + 1;  // no other workgroup of this launch reads k
 }
 
 // second reduction level for many tiles: workgroup b sums a contiguous slice
@@ -332,8 +336,10 @@ __global__ __launch_bounds__(kBlock) void cg_update_r(CgParams P) {
     CgState* st = P.st;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int stop = st->stop;
-    if (L == 0 && threadIdx.x == 0) st->stop_k2 = stop;
-    if (stop) return;
+    if (stop) {
+        if (L == 0 && threadIdx.x == 0) st->stop_k2 = stop;
+        return;
+    }
     const double alpha = cg_alpha_of(P, st->k, red);
     const int stride = gridDim.x * kBlock;
     double rr = 0.0, rz = 0.0;
@@ -349,6 +355,7 @@ __global__ __launch_bounds__(kBlock) void cg_update_r(CgParams P) {
     if (threadIdx.x == 0) {
         P.partB[L] = rr;
         P.partC[L] = rz;
+        if (L == 0) st->stop_k2 = 0;  // stored last, after every load of this workgroup
     }
 }
 
@@ -515,7 +522,6 @@ __global__ __launch_bounds__(kPanelBlock) void cg_spmv_panels(CgParams P) {
     __shared__ double red[kPanelBlock / 64];
     CgState* st = P.st;
     if (st->stop) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->k = st->k + 1;  // no other workgroup of this launch reads k
     const int blk = blockIdx.x;
     const double* __restrict__ p = P.p;
     panel_rows<VT>(P.pan, P.n, blk, P.dsplit, p, plds);
@@ -528,6 +534,9 @@ __global__ __launch_bounds__(kPanelBlock) void cg_spmv_panels(CgParams P) {
     }
     pap = block_sum_w<kPanelBlock / 64>(pap, red);
     if (threadIdx.x == 0) P.partA[blk] = pap;
+    // k advances after the last load: a store ahead of the loads could alias
+    // the tables and would turn their scalar loads into vector loads
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->k = st->k + 1;  // no other workgroup of this launch reads k
 }
 
 template <typename VT>
@@ -620,7 +629,6 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_dia(CgParams P) {
     const int stop = st->stop;
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     if (stop) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->k = st->k + 1;  // no other workgroup of this launch reads k
     double pap = 0.0;
 #pragma unroll
     for (int q = 0; q < kDiaCgRows / kBlock; ++q) {
@@ -634,6 +642,9 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_dia(CgParams P) {
     }
     pap = wave_sum(pap);
     if ((threadIdx.x & 63) == 0) P.partA[b * kWaves + (threadIdx.x >> 6)] = pap;
+    // k advances after the last load: a store ahead of the loads could alias
+    // the tables and would turn their scalar loads into vector loads
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->k = st->k + 1;  // no other workgroup of this launch reads k
 }
 
 // Diagonal values and masks from the CSR, one lane per row: entry k of row i
