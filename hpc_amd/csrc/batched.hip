@@ -60,7 +60,7 @@ __device__ __forceinline__ double system_sum(const double* part, int t0, int t1,
 template <int R>
 __global__ __launch_bounds__(kMaxBlock) void bsetup(BatchParams P) {
     __shared__ double red[kMaxBlock];
-    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int t = (int)blockIdx.x;
     const int j = threadIdx.x % R, g = threadIdx.x / R, G = blockDim.x / R;
     const int r0 = P.tile_row[t], r1 = P.tile_row[t + 1];
     double dot = 0.0, rz = 0.0;
@@ -229,7 +229,7 @@ template <typename VT, int R, bool DIA>
 __global__ __launch_bounds__(kMaxBlock) void bspmm_dot(BatchParams P) {
     constexpr int Q = SpmmLanes<R>::Q, L = SpmmLanes<R>::L;
     __shared__ double red[kMaxBlock * Q];
-    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int t = (int)blockIdx.x;
     const int s = P.tile_sys[t];
     BDiaSys Y;
     if (DIA) Y = P.dsys[s];
@@ -273,7 +273,7 @@ template <typename VT, int R, bool DIA>
 __global__ __launch_bounds__(kMaxBlock) void bspmm(BatchParams P, const double* __restrict__ xin,
                                                    double* __restrict__ y, const double* __restrict__ b) {
     constexpr int Q = SpmmLanes<R>::Q, L = SpmmLanes<R>::L;
-    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int t = (int)blockIdx.x;
     BDiaSys Y;
     if (DIA) Y = P.dsys[P.tile_sys[t]];
     const int l = threadIdx.x % L, g = threadIdx.x / L, G = blockDim.x / L;
@@ -315,7 +315,7 @@ __global__ __launch_bounds__(kMaxBlock) void bsys_alpha(BatchParams P) {
 template <int R>
 __global__ __launch_bounds__(kMaxBlock) void bupdate_r(BatchParams P) {
     __shared__ double red[kMaxBlock];
-    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int t = (int)blockIdx.x;
     const int s = P.tile_sys[t];
     if (!P.sys_active[s]) return;
     const int j = threadIdx.x % R, g = threadIdx.x / R, G = blockDim.x / R;
@@ -391,7 +391,7 @@ __global__ __launch_bounds__(kMaxBlock) void bsys_beta(BatchParams P) {
 
 template <int R>
 __global__ __launch_bounds__(kMaxBlock) void bupdate_xp(BatchParams P) {
-    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int t = (int)blockIdx.x;
     const int s = P.tile_sys[t];
     if (!P.sys_upd[s]) return;
     const int j = threadIdx.x % R, g = threadIdx.x / R, G = blockDim.x / R;
@@ -420,7 +420,7 @@ template <int R>
 __global__ __launch_bounds__(kMaxBlock) void btile_norm(BatchParams P, const double* __restrict__ v,
                                                         double* __restrict__ part) {
     __shared__ double red[kMaxBlock];
-    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int t = (int)blockIdx.x;
     const int j = threadIdx.x % R, g = threadIdx.x / R, G = blockDim.x / R;
     double s = 0.0;
     for (int i = P.tile_row[t] + g; i < P.tile_row[t + 1]; i += G) {
@@ -445,7 +445,7 @@ __global__ __launch_bounds__(kMaxBlock) void bsys_norm(BatchParams P, const doub
 template <int R>
 __global__ __launch_bounds__(kMaxBlock) void bmask(BatchParams P, const int* __restrict__ keep,
                                                    double* __restrict__ v) {
-    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int t = (int)blockIdx.x;
     const int s = P.tile_sys[t];
     const int j = threadIdx.x % R, g = threadIdx.x / R, G = blockDim.x / R;
     if (keep[s * R + j]) return;

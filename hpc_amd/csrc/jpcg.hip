@@ -173,7 +173,7 @@ __global__ __launch_bounds__(kBlock) void jpcg_persistent(JpcgParams P) {
 
 __global__ __launch_bounds__(kBlock) void cg_setup(CgParams P) {
     __shared__ double red[kWaves];
-    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int L = (int)blockIdx.x;
     const int stride = gridDim.x * kBlock;
     double dot = 0.0, rz = 0.0;
     for (int i = L * kBlock + threadIdx.x; i < P.n; i += stride) {
@@ -233,7 +233,7 @@ __global__ __launch_bounds__(kBlock) void cg_spmv(CgParams P) {
     __shared__ double red[kWaves];
     CgState* st = P.st;
     const int stop = st->stop;
-    const int w = xcd_remap(blockIdx.x, gridDim.x);
+    const int w = (int)blockIdx.x;
     const double* __restrict__ p = P.p;
     const CT* __restrict__ col = static_cast<const CT*>(P.col);
     const VT* __restrict__ val = static_cast<const VT*>(P.val);
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_pattern(CgParams P) {
     // the stop flag, the tiles' loads and this lane's p[row] are all issued
     // before the workgroup waits on any of them
     const int stop = st->stop;
-    const int w = xcd_remap(blockIdx.x, gridDim.x);
+    const int w = (int)blockIdx.x;
     const double* __restrict__ p = P.p;
     const VT* __restrict__ val = static_cast<const VT*>(P.val);
     PatternTile<VT> T[kPatternTiles];
@@ -334,7 +334,7 @@ __device__ __forceinline__ double cg_alpha_of(const CgParams& P, uint32_t k, dou
 __global__ __launch_bounds__(kBlock) void cg_update_r(CgParams P) {
     __shared__ double red[kWaves];
     CgState* st = P.st;
-    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int L = (int)blockIdx.x;
     const int stop = st->stop;
     if (stop) {
         if (L == 0 && threadIdx.x == 0) st->stop_k2 = stop;
@@ -363,7 +363,7 @@ __global__ __launch_bounds__(kBlock) void cg_update_xp(CgParams P) {
     __shared__ double red[kWaves];
     CgState* st = P.st;
     if (st->stop_k2) return;
-    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int L = (int)blockIdx.x;
     const uint32_t k = st->k;
     const double rz = st->rz[k & 1];
     const double alpha = cg_alpha_of(P, k, red);
@@ -410,7 +410,7 @@ __global__ __launch_bounds__(kBlock) void spmv_tiles(int ntiles, int nnz, const 
                                                      const double* __restrict__ b) {
     __shared__ double lds[kTileNnz];
     __shared__ double red[kWaves];
-    const int w = xcd_remap(blockIdx.x, gridDim.x);
+    const int w = (int)blockIdx.x;
     RowTile<VT, CT> T[kRowTiles];
 #pragma unroll
     for (int u = 0; u < kRowTiles; ++u) {
@@ -548,7 +548,7 @@ __global__ __launch_bounds__(kBlock) void spmv_pattern(int ntiles, int nnz, cons
                                                        double* __restrict__ y, const double* __restrict__ b) {
     __shared__ double lds[kTileNnz];
     __shared__ int loffs[32];
-    const int w = xcd_remap(blockIdx.x, gridDim.x);
+    const int w = (int)blockIdx.x;
     PatternTile<VT> T[kPatternTiles];
 #pragma unroll
     for (int u = 0; u < kPatternTiles; ++u) {
@@ -612,7 +612,7 @@ template <typename VT>
 __global__ __launch_bounds__(kBlock) void spmv_dia(int n, DiaDev D, const double* __restrict__ dsplit,
                                                    const double* __restrict__ x, double* __restrict__ y,
                                                    const double* __restrict__ b) {
-    const int i = xcd_remap(blockIdx.x, gridDim.x) * kBlock + (int)threadIdx.x;
+    const int i = (int)blockIdx.x * kBlock + (int)threadIdx.x;
     if (i >= n) return;
     const double s = dia_row<VT>(D, n, i, dsplit, x);
     y[i] = b ? b[i] - s : s;
@@ -627,7 +627,7 @@ template <typename VT>
 __global__ __launch_bounds__(kBlock) void cg_spmv_dia(CgParams P) {
     CgState* st = P.st;
     const int stop = st->stop;
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int b = (int)blockIdx.x;
     if (stop) return;
     double pap = 0.0;
 #pragma unroll
@@ -840,7 +840,7 @@ __global__ __launch_bounds__(kBlock) void sddmm_tiles(int ntiles, const int* __r
                                                       const double* __restrict__ lam,
                                                       const double* __restrict__ x, double* __restrict__ g) {
     __shared__ int srow[kTileNnz];
-    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int t = (int)blockIdx.x;
     if (t >= ntiles) return;
     const int r0 = tile_row[t], r1 = tile_row[t + 1];
     const int k0 = rowptr[r0], k1 = rowptr[r1];

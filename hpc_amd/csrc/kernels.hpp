@@ -28,8 +28,13 @@ struct alignas(64) Record {
 static_assert(sizeof(Record) == 64, "record layout");
 
 // Block index -> logical index so that each XCD (workgroups are dealt to the
-// 8 XCDs round-robin) receives one contiguous range of logical blocks, keeping
-// the x-vector gathers of neighbouring row tiles in one XCD's L2.
+// 8 XCDs round-robin) receives one contiguous range of logical blocks.  Only
+// the persistent loop's tile sweep uses it.  The launch-per-step kernels take
+// blocks in launch order: measured on MI355X, the contiguous ranges put eight
+// distant fronts of the matrix streams on HBM at once (the 160^3 diagonal SpMV
+// ran 3% slower) and leave an XCD idle once its own range of a batch has
+// converged (the batched iteration ran 29% slower); no kernel measured gained
+// from it.
 __device__ __forceinline__ int xcd_remap(int b, int G) {
     const int q = G >> 3, r = G & 7;
     const int x = b & 7, i = b >> 3;
