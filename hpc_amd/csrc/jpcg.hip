@@ -568,77 +568,122 @@ __global__ __launch_bounds__(kBlock) void spmv_pattern(int ntiles, int nnz, cons
 // --------------------------------------------------------------------------
 // diagonal layout
 
-// Row i of A times x on the diagonal layout, sixteen diagonals at a time:
-// the zero-padded value of each diagonal and x at the entry's column
-// (clamped into [0, n) for a diagonal the row does not have) are loaded
-// together with the mask, then the products of the row's entries are added
-// in ascending offset order, each rounded before it is added, left to right
-// from dsplit[i]*x[i] (or 0) -- the arithmetic of the row tiles.  Value loads
-// and the x reads of one offset are contiguous across the lanes of a wave.
+// Rows i0 .. i0+R-1 of A times x on the diagonal layout (R = 2 for fp64
+// values, 1 for fp32), C diagonals at a time (R * C = 16): the zero-padded
+// values of each diagonal (one R-wide load: the arrays have an even leading
+// dimension and i0 is a multiple of R) and x at the entries' columns (clamped
+// into [0, n) for a diagonal a row does not have) are loaded together with
+// the masks, then each row's products are added in ascending offset order,
+// each rounded before it is added, left to right from dsplit[i]*x[i] (or 0)
+// -- the arithmetic of the row tiles, row by row.  Value loads and the x
+// reads of one offset are contiguous across the lanes of a wave.  A padding
+// row (i0 + r == n when n is odd) reads in-bounds copies of row n-1's
+// operands and has mask 0; callers drop its sum.
 template <typename VT>
-__device__ __forceinline__ double dia_row(const DiaDev& D, int n, int i, const double* __restrict__ dsplit,
-                                          const double* __restrict__ x) {
+struct DiaLane {
+    static constexpr int R = sizeof(VT) == 8 ? 2 : 1;
+    static constexpr int C = 16 / R;
+};
+static_assert(DiaLane<double>::R == dia_rows_per_lane(false) && DiaLane<float>::R == dia_rows_per_lane(true),
+              "rows per lane agree with the host's grid");
+static_assert(kDiaBlock == kBlock, "diagonal kernels run kBlock lanes");
+
+template <typename T, int R>
+__device__ __forceinline__ void ld_rows(const T* p, T (&o)[R]) {
+    if constexpr (R == 1) {
+        o[0] = ld_stream(p);
+    } else {
+        typedef T V __attribute__((ext_vector_type(R)));
+        const V v = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
+#pragma unroll
+        for (int r = 0; r < R; ++r) o[r] = v[r];
+    }
+}
+
+template <typename VT>
+__device__ __forceinline__ void dia_rows(const DiaDev& D, int n, int i0, const double* __restrict__ dsplit,
+                                         const double* __restrict__ x, double (&s)[DiaLane<VT>::R]) {
 #pragma clang fp contract(off)
+    constexpr int R = DiaLane<VT>::R, C = DiaLane<VT>::C;
     const VT* __restrict__ val = static_cast<const VT*>(D.val);
     const int* __restrict__ off = D.off;
     const int nd = D.ndia;
-    const uint32_t m = D.mask[i];
-    double s = dsplit ? dsplit[i] * x[i] : 0.0;
-#pragma nounroll
-    for (int d0 = 0; d0 < nd; d0 += 16) {
-        VT v[16];
-        double g[16];
+    const size_t ld = (size_t)D.ld;
+    uint32_t m[R];
+    ld_rows<uint32_t, R>(D.mask + i0, m);
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
+    for (int r = 0; r < R; ++r) {
+        const int i = i0 + r < n ? i0 + r : n - 1;
+        s[r] = dsplit ? dsplit[i] * x[i] : 0.0;
+    }
+#pragma nounroll
+    for (int d0 = 0; d0 < nd; d0 += C) {
+        VT v[C][R];
+        double g[C][R];
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
             const int d = d0 + u;
             if (d < nd) {
-                v[u] = ld_stream(val + (size_t)d * n + i);
-                int c = i + off[d];
-                c = c < 0 ? 0 : (c >= n ? n - 1 : c);
-                g[u] = x[c];
+                ld_rows<VT, R>(val + (size_t)d * ld + i0, v[u]);
+                const int o = off[d];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    int c = i0 + r + o;
+                    c = c < 0 ? 0 : (c >= n ? n - 1 : c);
+                    g[u][r] = x[c];
+                }
             }
         }
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
+        for (int u = 0; u < C; ++u) {
             const int d = d0 + u;
-            if (d < nd && ((m >> d) & 1u)) s += (double)v[u] * g[u];
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (d < nd && ((m[r] >> d) & 1u)) s[r] += (double)v[u][r] * g[u][r];
         }
     }
-    return s;
 }
 
-// y = A x (or b - A x), one lane per row, 256 consecutive rows per workgroup
+// y = A x (or b - A x), R consecutive rows per lane, 256 R rows per workgroup
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void spmv_dia(int n, DiaDev D, const double* __restrict__ dsplit,
                                                    const double* __restrict__ x, double* __restrict__ y,
                                                    const double* __restrict__ b) {
-    const int i = (int)blockIdx.x * kBlock + (int)threadIdx.x;
-    if (i >= n) return;
-    const double s = dia_row<VT>(D, n, i, dsplit, x);
-    y[i] = b ? b[i] - s : s;
+    constexpr int R = DiaLane<VT>::R;
+    const int i0 = ((int)blockIdx.x * kBlock + (int)threadIdx.x) * R;
+    if (i0 >= n) return;
+    double s[R];
+    dia_rows<VT>(D, n, i0, dsplit, x, s);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (i0 + r < n) y[i0 + r] = b ? b[i0 + r] - s[r] : s[r];
 }
 
-// CG SpMV on the diagonal layout: kDiaCgRows consecutive rows per workgroup,
-// lane l taking rows l, l + 256, ... of the block (every address follows from
-// the block index, so nothing waits on a table); Ap is stored and each
-// wavefront leaves one p.Ap partial, its rows' products summed in a fixed
-// order (pass by pass, xor-butterfly within the wavefront) -- no barrier.
+// CG SpMV on the diagonal layout: 256 R consecutive rows per workgroup, R per
+// lane (every address follows from the block index, so nothing waits on a
+// table); Ap is stored and each wavefront leaves one p.Ap partial, a lane's
+// rows' products added in row order, then an xor-butterfly within the
+// wavefront -- a fixed order, no barrier.
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void cg_spmv_dia(CgParams P) {
+    constexpr int R = DiaLane<VT>::R;
     CgState* st = P.st;
     const int stop = st->stop;
     const int b = (int)blockIdx.x;
     if (stop) return;
     double pap = 0.0;
+    const int i0 = (b * kBlock + (int)threadIdx.x) * R;
+    if (i0 < P.n) {
+        double pi[R], y[R];
 #pragma unroll
-    for (int q = 0; q < kDiaCgRows / kBlock; ++q) {
-        const int i = b * kDiaCgRows + q * kBlock + (int)threadIdx.x;
-        if (i < P.n) {
-            const double pi = P.p[i];
-            const double y = dia_row<VT>(P.dia, P.n, i, P.dsplit, P.p);
-            P.Ap[i] = y;
-            pap += pi * y;
-        }
+        for (int r = 0; r < R; ++r) pi[r] = P.p[i0 + r < P.n ? i0 + r : P.n - 1];
+        dia_rows<VT>(P.dia, P.n, i0, P.dsplit, P.p, y);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (i0 + r < P.n) {
+                P.Ap[i0 + r] = y[r];
+                pap += pi[r] * y[r];
+            }
     }
     pap = wave_sum(pap);
     if ((threadIdx.x & 63) == 0) P.partA[b * kWaves + (threadIdx.x >> 6)] = pap;
@@ -648,8 +693,9 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_dia(CgParams P) {
 }
 
 // Diagonal values and masks from the CSR, one lane per row: entry k of row i
-// lands in diagonal d with off[d] = col[k] - i.  dval is zeroed beforehand
-// (a refresh of the values rewrites the same slots).
+// lands in diagonal d with off[d] = col[k] - i, at dval[d * dia_ld(n) + i].
+// dval and dmask are zeroed beforehand (a refresh of the values rewrites the
+// same slots; the padding row keeps mask 0).
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void fill_dia(int n, const int* __restrict__ rowptr, const int* __restrict__ col,
                                                    const VT* __restrict__ val, int ndia, const int* __restrict__ off,
@@ -676,7 +722,7 @@ __global__ __launch_bounds__(kBlock) void fill_dia(int n, const int* __restrict_
             }
             prev = lo;
             m |= 1u << lo;
-            dval[(size_t)lo * n + i] = val[k];
+            dval[(size_t)lo * dia_ld(n) + i] = val[k];
         }
         dmask[i] = m;
         if (bad) atomicOr(fail, 1);
@@ -926,7 +972,7 @@ hipError_t launch_cg_iteration(const CgParams& P, bool f32, bool c16, hipStream_
         else
             cg_spmv_panels<double><<<P.pan.nb, kPanelBlock, lds, s>>>(P);
     } else if (P.dia_on) {
-        const int g = (P.n + kDiaCgRows - 1) / kDiaCgRows;
+        const int g = dia_groups(P.n, f32);
         if (f32)
             cg_spmv_dia<float><<<g, kBlock, 0, s>>>(P);
         else
@@ -955,7 +1001,7 @@ hipError_t launch_spmv(const CsrDev& A, const double* x, double* y, const double
         return hipGetLastError();
     }
     if (A.dia_on) {
-        const int g = (A.n + kBlock - 1) / kBlock;
+        const int g = dia_groups(A.n, A.val_f32);
         if (g <= 0) return hipSuccess;
         if (A.val_f32)
             spmv_dia<float><<<g, kBlock, 0, s>>>(A.n, A.dia, A.dsplit, x, y, b);

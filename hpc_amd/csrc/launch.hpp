@@ -34,13 +34,26 @@ struct PanelDev {
 // value array per global column offset col - row, ascending, with 0 where a
 // row has no entry there, and a presence mask per row.
 constexpr int kDiaMax = 32;
-constexpr int kDiaCgRows = 256;     // rows per workgroup of the diagonal layout's CG SpMV (4 partials each)
-constexpr int dia_cg_partials(int n) { return (n + kDiaCgRows - 1) / kDiaCgRows * 4; }
+// Rows per lane of the diagonal kernels: two consecutive rows for fp64 values
+// (one 16-byte load per diagonal), one row for fp32 values.
+constexpr int dia_rows_per_lane(bool f32) { return f32 ? 1 : 2; }
+// Leading dimension of the diagonal value arrays and length of the mask
+// array: n rounded up to even, so a lane's two rows are one aligned 16-byte
+// load; the padding row's mask is 0 and its values are 0.
+constexpr int dia_ld(int n) { return (n + 1) & ~1; }
+// workgroups of the diagonal kernels (kDiaBlock lanes each) and the CG
+// SpMV's p.Ap partials (one per wavefront)
+constexpr int kDiaBlock = 256;
+constexpr int dia_groups(int n, bool f32) {
+    return (n + kDiaBlock * dia_rows_per_lane(f32) - 1) / (kDiaBlock * dia_rows_per_lane(f32));
+}
+constexpr int dia_cg_partials(int n, bool f32) { return dia_groups(n, f32) * (kDiaBlock / 64); }
 struct DiaDev {
     int ndia = 0;
+    int ld = 0;                      // dia_ld(n)
     const int* off = nullptr;        // [ndia] offsets, ascending
-    const void* val = nullptr;       // [ndia][n] in the operator's value type
-    const uint32_t* mask = nullptr;  // [n]: bit d set when row i has an entry at off[d]
+    const void* val = nullptr;       // [ndia][ld] in the operator's value type
+    const uint32_t* mask = nullptr;  // [ld]: bit d set when row i has an entry at off[d]
 };
 
 struct CsrDev {

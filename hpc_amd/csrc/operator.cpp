@@ -2,7 +2,6 @@
 #include "operator.hpp"
 
 #include <algorithm>
-#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -233,60 +232,53 @@ Operator::Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t*
             A_.col16 = col16_.ptr;
         }
     }
-    // Layout choices, timed on this operator: column-offset pattern tiles when
-    // every tile has at most 32 distinct offsets (stencils), the diagonal
-    // layout when the whole matrix has at most 32 (and its value arrays stay
-    // within twice the stored entries), and, when the columns are too spread
-    // for the int16 stream and x is far larger than an XCD's L2, the panel
-    // layout (only when at least 15% faster than the tiles).  The standalone
-    // SpMV keeps the layout with the fastest SpMV launches; the solves keep
-    // the one with the fastest short CG run among those, which can differ
-    // (a CG iteration adds the Ap store, p.Ap partials and the vector kernels).
+    // Layouts for the standalone SpMV, timed on this operator: column-offset
+    // pattern tiles when every tile has at most 32 distinct offsets
+    // (stencils), the diagonal layout when the whole matrix has at most 32
+    // (and its value arrays stay within twice the stored entries), and, when
+    // the columns are too spread for the int16 stream and x is far larger than
+    // an XCD's L2, the panel layout (only when at least 15% faster than the
+    // tiles).  Every layout sums each row left to right, so the SpMV result
+    // does not depend on the choice.
+    //
+    // The solves' layout follows from the matrix instead, so that operators
+    // built from the same matrix give bit-identical iterates whatever the
+    // timings: the diagonals when they hold at most 25% padding slots, else
+    // the panels when accepted, else the tiles or the pattern tiles (whichever
+    // SpMV is faster; both give the same per-tile p.Ap partials, so the same
+    // iterates).  The diagonal and panel layouts group the p.Ap partials their
+    // own way (per wavefront, per row block).
     if (nnz > 0) {
-        std::vector<int> cand(1, kTiles);
         const float t_tiles = time_layout(kTiles, s);
-        float best = t_tiles;
+        float best = t_tiles, t_pat = 0.0f;
         if (build_patterns(s)) {
-            cand.push_back(kPatterns);
-            const float t_pat = time_layout(kPatterns, s);
+            t_pat = time_layout(kPatterns, s);
             if (t_pat > 0.0f && t_pat < best) {
                 best = t_pat;
                 auto_layout_ = kPatterns;
             }
         }
         if (build_diagonals(s)) {
-            cand.push_back(kDiagonals);
             const float t_dia = time_layout(kDiagonals, s);
             if (t_dia > 0.0f && t_dia < best) {
                 best = t_dia;
                 auto_layout_ = kDiagonals;
             }
         }
+        bool panels_ok = false;
         if (!has_col16_ && n >= kPanelAutoRows && build_panels(s)) {
             const float t_pan = time_layout(kPanels, s);
             if (t_pan > 0.0f && t_pan < 0.85f * t_tiles) {
-                cand.push_back(kPanels);
+                panels_ok = true;
                 if (t_pan < best) {
                     best = t_pan;
                     auto_layout_ = kPanels;
                 }
             }
         }
-        auto_solve_ = auto_layout_;
-        if (cand.size() > 1) {
-            DevBuf<double> tmp;
-            tmp.ensure((size_t)3 * n);
-            hip_check(launch_fill(3 * (int64_t)n, 1.0, tmp.ptr, s), "timing vectors");
-            double tbest = 0.0;
-            for (int L : cand) {
-                const double t = time_solve_layout(L, tmp.ptr, s);
-                if (tbest == 0.0 || t < tbest) {
-                    tbest = t;
-                    auto_solve_ = L;
-                }
-            }
-            drop_graph();  // captured with the timing vectors
-        }
+        auto_solve_ = (t_pat > 0.0f && t_pat < t_tiles) ? kPatterns : kTiles;
+        if (panels_ok) auto_solve_ = kPanels;
+        if (has_dia_ && 4 * (int64_t)A_.dia.ndia * n <= 5 * nnz) auto_solve_ = kDiagonals;
         const auto used = [&](int L) { return auto_layout_ == L || auto_solve_ == L; };
         if (!used(kPanels)) free_panels();
         if (!used(kPatterns)) free_patterns();
@@ -386,11 +378,12 @@ bool Operator::build_diagonals(hipStream_t s) {
     }
     const int ndia = (int)offs.size();
     if (ndia == 0 || ndia > kDiaMax || (int64_t)ndia * A_.n > 2 * A_.nnz) return false;
-    const int nblk = dia_cg_partials(A_.n);  // the CG kernel's p.Ap partials
-    const size_t vbytes = (size_t)ndia * A_.n * (A_.val_f32 ? 4 : 8);
+    const int nblk = dia_cg_partials(A_.n, A_.val_f32);  // the CG kernel's p.Ap partials
+    const int ld = dia_ld(A_.n);
+    const size_t vbytes = (size_t)ndia * ld * (A_.val_f32 ? 4 : 8);
     try {
         doff_.ensure(kDiaMax + 8);  // padded: lanes never read past the table
-        dmask_.ensure((size_t)A_.n);
+        dmask_.ensure((size_t)ld);
         dval_.ensure((vbytes + 7) / 8);
         partA_.ensure((size_t)std::max({A_.ntiles, nblk, A_.pan.nb}));
     } catch (const Error& e) {
@@ -403,6 +396,7 @@ bool Operator::build_diagonals(hipStream_t s) {
     hip_check(hipMemcpyAsync(doff_.ptr, offs.data(), sizeof(int32_t) * offs.size(), hipMemcpyHostToDevice, s),
               "upload offsets");
     hip_check(hipMemsetAsync(dval_.ptr, 0, vbytes, s), "clear diagonals");
+    hip_check(hipMemsetAsync(dmask_.ptr, 0, sizeof(uint32_t) * ld, s), "clear masks");
     hip_check(hipMemsetAsync(abort_.ptr, 0, sizeof(int), s), "reset flag");
     hip_check(launch_fill_dia(A_, ndia, doff_.ptr, dval_.ptr, dmask_.ptr, abort_.ptr, s), "fill diagonals");
     int fail = 1;
@@ -414,6 +408,7 @@ bool Operator::build_diagonals(hipStream_t s) {
         return false;
     }
     A_.dia.ndia = ndia;
+    A_.dia.ld = ld;
     A_.dia.off = doff_.ptr;
     A_.dia.val = dval_.ptr;
     A_.dia.mask = dmask_.ptr;
@@ -501,23 +496,6 @@ void Operator::refresh_panels(hipStream_t s) const {
     if (!(A_.pan_on || solve_layout_ == kPanels) || !pan_stale_) return;
     hip_check(launch_gather_vals(A_.nnz, pperm_.ptr, A_.val, pval_.ptr, A_.val_f32, s), "panel values");
     pan_stale_ = false;
-}
-
-// Wall time of a short CG run on `layout` (24 rounds: three graph chunks,
-// after one untimed run that captures the graph) with b = 1 and a unit Jacobi
-// diagonal; v holds 3n doubles (x, b, diagonal) with b and the diagonal = 1.
-double Operator::time_solve_layout(int layout, double* v, hipStream_t s) {
-    const int saved = solve_layout_;
-    solve_layout_ = layout;
-    drop_graph();
-    refresh_copies(s);
-    const size_t n = (size_t)A_.n;
-    (void)solve_multi(v + 2 * n, v + n, v, 23, 0.0, nullptr, s);
-    const auto t0 = std::chrono::steady_clock::now();
-    (void)solve_multi(v + 2 * n, v + n, v, 23, 0.0, nullptr, s);
-    const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    solve_layout_ = saved;
-    return t;
 }
 
 // time 5 SpMV launches on `layout` (after 2 warm-up launches); leaves that layout switched on
@@ -654,7 +632,7 @@ CgParams Operator::cg_params(const double* diag, const double* b, double* x, uin
     P.tiles = A_.ntiles;
     // p.Ap partials: one per panel row block, per wavefront of the diagonal
     // layout's workgroups, or per row tile
-    P.ntiles = P.pan_on ? A_.pan.nb : P.dia_on ? dia_cg_partials(A_.n) : A_.ntiles;
+    P.ntiles = P.pan_on ? A_.pan.nb : P.dia_on ? dia_cg_partials(A_.n, A_.val_f32) : A_.ntiles;
     P.nnz = (int)A_.nnz;
     P.G2 = G2_;
     P.split_alpha = P.ntiles > kMaxRedundantPartials ? 1 : 0;

@@ -177,7 +177,6 @@ class Operator {
     void refresh_copies(hipStream_t s) const;
     void copy_pattern_to_host(hipStream_t s, std::vector<int32_t>& rp, std::vector<int32_t>& col) const;
     float time_layout(int layout, hipStream_t s);
-    double time_solve_layout(int layout, double* v, hipStream_t s);
     void free_panels();
     void free_patterns();
     void refresh_panels(hipStream_t s) const;

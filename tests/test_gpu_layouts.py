@@ -145,7 +145,8 @@ def test_pattern_tiles_bitexact_and_identical_solves(cuda, kind):
         sols[layout] = (xs.cpu().numpy(), info.executed)
     # identical SpMV and identical per-tile p.Ap partials: identical solves
     assert sols[1][1] == sols[3][1] and np.array_equal(sols[1][0], sols[3][0])
-    # diagonals: the same SpMV, p.Ap partials per 256 rows instead of per tile -- deterministic, oracle parity
+    # diagonals: the same SpMV, p.Ap partials per wavefront (128 rows for fp64) instead of per tile --
+    # deterministic, oracle parity
     xs2, info2 = op.solve(diag, b, s.max_iter, s.tol)
     assert info2.executed == sols[4][1] and np.array_equal(xs2.cpu().numpy(), sols[4][0])
     o = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, s.b, s.max_iter, s.tol)
@@ -296,6 +297,25 @@ def test_diagonals_value_refresh_and_boundary_rows(cuda):
     w = torch.from_numpy(np.ascontiguousarray(val * 3.0)).to(cuda)
     op.set_values(w)
     assert np.array_equal(op.spmv(x).cpu().numpy(), oracle.spmv(rp, col, 3.0 * val, xv))
+
+
+def test_diagonal_solve_odd_rows(cuda):
+    """Diagonal-layout CG on an odd number of rows: the fp64 kernels take two rows per lane, so the last
+    lane's second row is padding (mask 0, sum dropped, no Ap store).  Oracle parity and repeatability."""
+    from hpc_amd import device
+    A = workloads.stencil(workloads.STENCIL_3D7, 21, 13, 7, [0.5, 1.5, 2.5])
+    assert A.n % 2 == 1
+    b = workloads.uniform(5, 2, A.n, -1, 1)
+    op = device.CsrOperator(_t(cuda, A.rowptr), _t(cuda, A.col), _t(cuda, A.val), rowptr_host=A.rowptr)
+    assert op.set_layout(4) == 4 and op.solve_layout == 4
+    diag, bt = _t(cuda, A.diag), _t(cuda, b)
+    x1, i1 = op.solve(diag, bt, 2000, 1e-12)
+    x2, i2 = op.solve(diag, bt, 2000, 1e-12)
+    assert i1.executed == i2.executed and bool((x1 == x2).all())
+    o = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, b, 2000, 1e-12)
+    _say(f"n={A.n} diagonals: {i1.executed} iterations, oracle {o['executed']}")
+    assert abs(i1.executed - o["executed"]) <= 1
+    assert np.linalg.norm(x1.cpu().numpy() - o["x"]) / np.linalg.norm(o["x"]) <= 1e-10
 
 
 def test_solve_layout_chosen_separately(cuda):
