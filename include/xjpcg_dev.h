@@ -67,8 +67,8 @@ XJPCG_Status_t xjpcg_op_set_values(xjpcg_op_t* op, const void* d_val);
  * most 32 distinct offsets col - row plus a 32-bit mask per row replace the
  * column stream, 4 bytes per row instead of 2-4 per entry; same sums as 1);
  * 4 = diagonals (one value array per global offset col - row with zeros where
- * a row has no entry, a 32-bit presence mask per row; one lane per row sums
- * its entries in column order: same sums as 1).
+ * a row has no entry, a 32-bit presence mask per row; each row's entries are
+ * summed in column order, two rows per lane for fp64 values: same sums as 1).
  * At creation the operator times the layouts that apply and keeps the fastest
  * (panels only when at least 15% faster than the tiles); layout = 0 restores
  * that choice, layout < 0 only queries.  Layout 2 needs each row's columns in
@@ -78,11 +78,13 @@ XJPCG_Status_t xjpcg_op_set_values(xjpcg_op_t* op, const void* d_val);
  * most twice the stored entries.  *h_active (may be NULL) receives the layout
  * in use.  Building a layout synchronises `stream`. */
 XJPCG_Status_t xjpcg_op_set_layout(xjpcg_op_t* op, int layout, void* stream, int* h_active);
-/* The layout the solves run on.  At creation the operator also times a short
- * CG run on each layout that applies and keeps the fastest for its solves,
- * which can differ from the standalone SpMV's layout (a CG iteration adds the
- * Ap store, the p.Ap partials and the vector kernels).  xjpcg_op_set_layout
- * with layout > 0 sets both; layout = 0 restores both creation-time choices. */
+/* The layout the solves run on.  It follows from the matrix, not from
+ * timing, so that operators built from the same matrix give bit-identical
+ * iterates: diagonals when they apply with at most 25% padding slots, else
+ * the panels when the SpMV kept them, else the tiles or the pattern tiles
+ * (the faster SpMV of the two; both give the same iterates).  It can differ
+ * from the standalone SpMV's layout.  xjpcg_op_set_layout with layout > 0
+ * sets both; layout = 0 restores both creation-time choices. */
 XJPCG_Status_t xjpcg_op_solve_layout(const xjpcg_op_t* op, int* h_layout);
 /* Host-only view of the panel layout an operator would build (no device
  * calls): xjpcg_panel_dims gives the rows per block, the row blocks and the
