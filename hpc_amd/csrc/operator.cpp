@@ -9,6 +9,8 @@
 
 namespace xj {
 
+constexpr int kDiaSolveRows = 1 << 19;  // smallest operator whose solves run on the diagonals
+
 // Workgroups of the persistent loop: every grid barrier costs each workgroup
 // an L2 write-back and invalidate on its XCD, so a small operator runs best on
 // far fewer workgroups than fit on the chip.  Per-iteration times measured on
@@ -243,7 +245,10 @@ Operator::Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t*
     //
     // The solves' layout follows from the matrix instead, so that operators
     // built from the same matrix give bit-identical iterates whatever the
-    // timings: the diagonals when they hold at most 25% padding slots, else
+    // timings: the diagonals when they hold at most 25% padding slots and the
+    // matrix has at least kDiaSolveRows rows (below that the fp64 kernel's
+    // 512-row workgroups leave the chip half empty: a 512^2 diffusion solve
+    // took 61.9 ms on them against 56.8 ms on the tiles), else
     // the panels when accepted, else the tiles or the pattern tiles (whichever
     // SpMV is faster; both give the same per-tile p.Ap partials, so the same
     // iterates).  The diagonal and panel layouts group the p.Ap partials their
@@ -278,7 +283,7 @@ Operator::Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t*
         }
         auto_solve_ = (t_pat > 0.0f && t_pat < t_tiles) ? kPatterns : kTiles;
         if (panels_ok) auto_solve_ = kPanels;
-        if (has_dia_ && 4 * (int64_t)A_.dia.ndia * n <= 5 * nnz) auto_solve_ = kDiagonals;
+        if (has_dia_ && n >= kDiaSolveRows && 4 * (int64_t)A_.dia.ndia * n <= 5 * nnz) auto_solve_ = kDiagonals;
         const auto used = [&](int L) { return auto_layout_ == L || auto_solve_ == L; };
         if (!used(kPanels)) free_panels();
         if (!used(kPatterns)) free_patterns();
