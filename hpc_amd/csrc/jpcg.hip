@@ -571,14 +571,16 @@ __global__ __launch_bounds__(kBlock) void spmv_pattern(int ntiles, int nnz, cons
 // Rows i0 .. i0+R-1 of A times x on the diagonal layout (R = 2 for fp64
 // values, 1 for fp32), C diagonals at a time (R * C = 16): the zero-padded
 // values of each diagonal (one R-wide load: the arrays have an even leading
-// dimension and i0 is a multiple of R) and x at the entries' columns (one
-// R-wide load per diagonal, clamped into [0, n) for a diagonal a row does not
-// have) are loaded together with the masks, then each row's products are added in ascending offset order,
+// dimension and i0 is a multiple of R) and x at the entries' columns (clamped
+// into [0, n) for a diagonal a row does not have) are loaded together with
+// the masks, then each row's products are added in ascending offset order,
 // each rounded before it is added, left to right from dsplit[i]*x[i] (or 0)
 // -- the arithmetic of the row tiles, row by row.  Value loads and the x
 // reads of one offset are contiguous across the lanes of a wave.  A padding
 // row (i0 + r == n when n is odd) reads in-bounds copies of row n-1's
-// operands and has mask 0; callers drop its sum.
+// operands and has mask 0; callers drop its sum.  dia_rows_s reads x one
+// element per row; dia_rows_v (R > 1, n >= R) reads a diagonal's R x values
+// with one load.
 template <typename VT>
 struct DiaLane {
     static constexpr int R = sizeof(VT) == 8 ? 2 : 1;
