@@ -318,6 +318,28 @@ def test_diagonal_solve_odd_rows(cuda):
     assert np.linalg.norm(x1.cpu().numpy() - o["x"]) / np.linalg.norm(o["x"]) <= 1e-10
 
 
+def test_equal_matrices_give_identical_solves(cuda):
+    """The solve layout follows from the matrix, not from creation-time timings: two operators built from
+    the same 81^3 27-point matrix (above the diagonal-solve size floor) both solve on the diagonals and give
+    bit-identical iterates, at oracle parity."""
+    from hpc_amd import device
+    s = workloads.c2_stencil27(81)
+    A = s.A
+    assert A.n >= 1 << 19
+    out = []
+    for _ in range(2):
+        op = device.CsrOperator(_t(cuda, A.rowptr), _t(cuda, A.col), _t(cuda, A.val), rowptr_host=A.rowptr)
+        assert op.solve_layout == 4
+        x, info = op.solve(_t(cuda, A.diag), _t(cuda, s.b), s.max_iter, 1e-8)
+        out.append((x.cpu().numpy(), info.executed, info.residual))
+        op.close()
+    assert out[0][1] == out[1][1] and out[0][2] == out[1][2] and np.array_equal(out[0][0], out[1][0])
+    o = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, s.b, s.max_iter, 1e-8)
+    _say(f"n={A.n}: {out[0][1]} iterations, oracle {o['executed']}")
+    assert abs(out[0][1] - o["executed"]) <= 1
+    assert np.linalg.norm(out[0][0] - o["x"]) / np.linalg.norm(o["x"]) <= 1e-10
+
+
 def test_solve_layout_chosen_separately(cuda):
     """The solves keep their own creation-time layout; set_layout(L > 0) sets both, 0 restores both."""
     from hpc_amd import device
