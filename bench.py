@@ -709,6 +709,10 @@ def main():
                      "algorithmic_bytes_per_launch": nb,
                      "achievable_copy_GBs": round(copy_gbs, 1),
                      "physical_GBs": round(traffic / (spmv_med * 1e-3) / 1e9, 1) if traffic else None,
+                     "physical_frac": round(traffic / (spmv_med * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+                     "bytes_basis": "algorithmic = CSR bytes 12*nnz + 4*(n+1) + 16*n (int32 columns); the "
+                                    "diagonal and pattern layouts move fewer bytes than that (no column stream), "
+                                    "so frac can reach 1; physical_* use the counter-measured L2 fill bytes",
                      "traffic_source": (traffic_src + " (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)")
                      if traffic_src else None},
         "cpu_baseline": cpu,
