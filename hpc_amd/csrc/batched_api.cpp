@@ -144,12 +144,45 @@ class Batch {
         return ms / 5.0f;
     }
 
-    // The autotuner: time the batched SpMM for each candidate (layout, tile
-    // rows, block) and keep the fastest.
+    // Mean time of a whole batched iteration (SpMM, per-system alpha, r
+    // update, beta, x/p update) with every pair running, in the current
+    // layout and tile configuration: set-up on b = 1 (held in r) and a unit
+    // diagonal with tolerance 0, 2 untimed iterations, then 5 timed.  x and
+    // the diagonal live in `v` (N*R + N doubles).
+    float time_iteration(double* v) {
+        double* x = v;
+        double* d = v + (size_t)N_ * R_;
+        hip_check(launch_fill((int64_t)N_ * R_, 1.0, r_.ptr, work_), "tune input");
+        BatchParams P = params(d, r_.ptr, x, 0.0, 0xFFFFFFF0u);
+        hip_check(hipMemsetAsync(any_.ptr, 0, sizeof(int), work_), "reset flag");
+        hip_check(batch_launch_setup(P, R_, block_, work_), "tune set-up");
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "event");
+        hip_check(hipEventCreate(&e1), "event");
+        for (int w = 0; w < 2; ++w) hip_check(batch_launch_iteration(P, R_, f32_, block_, work_), "tune");
+        hip_check(hipEventRecord(e0, work_), "event");
+        for (int w = 0; w < 5; ++w) hip_check(batch_launch_iteration(P, R_, f32_, block_, work_), "tune");
+        hip_check(hipEventRecord(e1, work_), "event");
+        hip_check(hipEventSynchronize(e1), "event");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "event");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return ms / 5.0f;
+    }
+
+    // The autotuner: time whole batched iterations for each candidate
+    // (layout, tile rows, block) and keep the fastest.  The tile and block
+    // sizes drive the vector kernels as well as the SpMM (together most of an
+    // iteration), so timing the SpMM alone picked shapes that slowed the
+    // iteration down.
     void autotune(hipStream_t user) {
         static const int tiles[] = {32, 64, 128, 256, 512};
         static const int blocks[] = {128, 256, 512};
         sync_user(user);
+        DevBuf<double> tmp;
+        tmp.ensure((size_t)N_ * R_ + (size_t)N_);
+        hip_check(launch_fill((int64_t)N_ * R_ + N_, 1.0, tmp.ptr, work_), "tune input");
         float best = 1e30f;
         int bt = tile_rows_, bb = block_;
         bool bd = dia_on_;
@@ -159,7 +192,7 @@ class Batch {
                 for (int bl : blocks) {
                     if (bl < R_ || bl / R_ > tr) continue;  // at most one row per lane group pass beyond the tile
                     configure(tr, bl, work_);
-                    const float ms = time_spmm();
+                    const float ms = time_iteration(tmp.ptr);
                     if (ms < best) {
                         best = ms;
                         bt = tr;
