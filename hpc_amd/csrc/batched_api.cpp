@@ -180,9 +180,18 @@ class Batch {
         static const int tiles[] = {32, 64, 128, 256, 512};
         static const int blocks[] = {128, 256, 512};
         sync_user(user);
+        // scratch x and diagonal for the timing iterations; when the device
+        // cannot hold them next to the batch, candidates are timed on the
+        // SpMM alone
         DevBuf<double> tmp;
-        tmp.ensure((size_t)N_ * R_ + (size_t)N_);
-        hip_check(launch_fill((int64_t)N_ * R_ + N_, 1.0, tmp.ptr, work_), "tune input");
+        bool whole = true;
+        try {
+            tmp.ensure((size_t)N_ * R_ + (size_t)N_);
+        } catch (const Error& e) {
+            if (e.status != XJPCG_STATUS_ALLOC_FAILED) throw;
+            whole = false;
+        }
+        if (whole) hip_check(launch_fill((int64_t)N_ * R_ + N_, 1.0, tmp.ptr, work_), "tune input");
         float best = 1e30f;
         int bt = tile_rows_, bb = block_;
         bool bd = dia_on_;
@@ -192,7 +201,7 @@ class Batch {
                 for (int bl : blocks) {
                     if (bl < R_ || bl / R_ > tr) continue;  // at most one row per lane group pass beyond the tile
                     configure(tr, bl, work_);
-                    const float ms = time_iteration(tmp.ptr);
+                    const float ms = whole ? time_iteration(tmp.ptr) : time_spmm();
                     if (ms < best) {
                         best = ms;
                         bt = tr;
