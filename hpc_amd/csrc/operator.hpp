@@ -49,6 +49,9 @@ struct DevBuf {
         hipError_t e = hipMalloc((void**)&ptr, n * sizeof(T));
         if (e != hipSuccess) {
             ptr = nullptr;
+            // callers may catch this and carry on: clear the error, or the
+            // next hipGetLastError (a launch check) would report it again
+            (void)hipGetLastError();
             throw alloc_failed("device allocation of " + std::to_string(n * sizeof(T)) + " bytes failed");
         }
         count = n;
