@@ -145,15 +145,17 @@ class Batch {
     }
 
     // Mean time of a whole batched iteration (SpMM, per-system alpha, r
-    // update, beta, x/p update) with every pair running, in the current
-    // layout and tile configuration: set-up on b = 1 (held in r) and a unit
-    // diagonal with tolerance 0, 2 untimed iterations, then 5 timed.  x and
+    // update, beta, x/p update) in the current layout and tile configuration:
+    // set-up on b = 1 (held in r) and a unit diagonal, 2 untimed iterations,
+    // then 5 timed.  busy: tolerance 0, every pair runs; otherwise an
+    // infinite tolerance, every pair stops in the first iteration and the
+    // timed ones are the bare launches of a batch that has converged.  x and
     // the diagonal live in `v` (N*R + N doubles).
-    float time_iteration(double* v) {
+    float time_iteration(double* v, bool busy) {
         double* x = v;
         double* d = v + (size_t)N_ * R_;
         hip_check(launch_fill((int64_t)N_ * R_, 1.0, r_.ptr, work_), "tune input");
-        BatchParams P = params(d, r_.ptr, x, 0.0, 0xFFFFFFF0u);
+        BatchParams P = params(d, r_.ptr, x, busy ? 0.0 : INFINITY, 0xFFFFFFF0u);
         hip_check(hipMemsetAsync(any_.ptr, 0, sizeof(int), work_), "reset flag");
         hip_check(batch_launch_setup(P, R_, block_, work_), "tune set-up");
         hipEvent_t e0, e1;
@@ -172,10 +174,14 @@ class Batch {
     }
 
     // The autotuner: time whole batched iterations for each candidate
-    // (layout, tile rows, block) and keep the fastest.  The tile and block
-    // sizes drive the vector kernels as well as the SpMM (together most of an
+    // (layout, tile rows, block) and keep the one with the lowest
+    // 0.25 * busy + 0.75 * idle iteration time.  The tile and block sizes
+    // drive the vector kernels as well as the SpMM (together most of an
     // iteration), so timing the SpMM alone picked shapes that slowed the
-    // iteration down.
+    // iteration down; and a batch spends most of its iterations with few
+    // pairs left (pair iterations 77..2184 in the 512-system workload), where
+    // the cost is launching one workgroup per tile: timing busy iterations
+    // alone once picked 32-row tiles (1.4M workgroups per launch) and lost 9%.
     void autotune(hipStream_t user) {
         static const int tiles[] = {32, 64, 128, 256, 512};
         static const int blocks[] = {128, 256, 512};
@@ -201,7 +207,8 @@ class Batch {
                 for (int bl : blocks) {
                     if (bl < R_ || bl / R_ > tr) continue;  // at most one row per lane group pass beyond the tile
                     configure(tr, bl, work_);
-                    const float ms = whole ? time_iteration(tmp.ptr) : time_spmm();
+                    const float ms = whole ? 0.25f * time_iteration(tmp.ptr, true) + 0.75f * time_iteration(tmp.ptr, false)
+                                           : time_spmm();
                     if (ms < best) {
                         best = ms;
                         bt = tr;
