@@ -11,6 +11,26 @@
 
 namespace xj {
 
+// Two timing events, destroyed on every path out of the scope.
+struct EventPair {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    EventPair() {
+        hip_check(hipEventCreate(&e0), "event");
+        hip_check(hipEventCreate(&e1), "event");
+    }
+    ~EventPair() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+    EventPair(const EventPair&) = delete;
+    EventPair& operator=(const EventPair&) = delete;
+    float ms() const {
+        float t = 0.f;
+        hip_check(hipEventElapsedTime(&t, e0, e1), "event");
+        return t;
+    }
+};
+
 class Batch {
   public:
     Batch(int S, int R, const int32_t* h_sys_row, const int32_t* h_rowptr, const int32_t* d_rowptr,
@@ -128,20 +148,14 @@ class Batch {
         double* xs = p_.ptr;
         double* ys = Ap_.ptr;
         hip_check(hipMemsetAsync(xs, 0, (size_t)N_ * R_ * sizeof(double), work_), "tune input");
-        hipEvent_t e0, e1;
-        hip_check(hipEventCreate(&e0), "event");
-        hip_check(hipEventCreate(&e1), "event");
+        EventPair ev;
         BatchParams P = params(nullptr, nullptr, nullptr, 0.0, 1);
         for (int w = 0; w < 2; ++w) hip_check(batch_launch_spmm(P, R_, f32_, block_, xs, ys, nullptr, work_), "tune");
-        hip_check(hipEventRecord(e0, work_), "event");
+        hip_check(hipEventRecord(ev.e0, work_), "event");
         for (int w = 0; w < 5; ++w) hip_check(batch_launch_spmm(P, R_, f32_, block_, xs, ys, nullptr, work_), "tune");
-        hip_check(hipEventRecord(e1, work_), "event");
-        hip_check(hipEventSynchronize(e1), "event");
-        float ms = 0.f;
-        hip_check(hipEventElapsedTime(&ms, e0, e1), "event");
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        return ms / 5.0f;
+        hip_check(hipEventRecord(ev.e1, work_), "event");
+        hip_check(hipEventSynchronize(ev.e1), "event");
+        return ev.ms() / 5.0f;
     }
 
     // Mean time of a whole batched iteration (SpMM, per-system alpha, r
@@ -158,19 +172,13 @@ class Batch {
         BatchParams P = params(d, r_.ptr, x, busy ? 0.0 : INFINITY, 0xFFFFFFF0u);
         hip_check(hipMemsetAsync(any_.ptr, 0, sizeof(int), work_), "reset flag");
         hip_check(batch_launch_setup(P, R_, block_, work_), "tune set-up");
-        hipEvent_t e0, e1;
-        hip_check(hipEventCreate(&e0), "event");
-        hip_check(hipEventCreate(&e1), "event");
+        EventPair ev;
         for (int w = 0; w < 2; ++w) hip_check(batch_launch_iteration(P, R_, f32_, block_, work_), "tune");
-        hip_check(hipEventRecord(e0, work_), "event");
+        hip_check(hipEventRecord(ev.e0, work_), "event");
         for (int w = 0; w < 5; ++w) hip_check(batch_launch_iteration(P, R_, f32_, block_, work_), "tune");
-        hip_check(hipEventRecord(e1, work_), "event");
-        hip_check(hipEventSynchronize(e1), "event");
-        float ms = 0.f;
-        hip_check(hipEventElapsedTime(&ms, e0, e1), "event");
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        return ms / 5.0f;
+        hip_check(hipEventRecord(ev.e1, work_), "event");
+        hip_check(hipEventSynchronize(ev.e1), "event");
+        return ev.ms() / 5.0f;
     }
 
     // The autotuner: time whole batched iterations for each candidate
