@@ -340,6 +340,23 @@ def test_equal_matrices_give_identical_solves(cuda):
     assert np.linalg.norm(out[0][0] - o["x"]) / np.linalg.norm(o["x"]) <= 1e-10
 
 
+def test_diagonal_solve_7point_above_floor(cuda):
+    """An anisotropic 3D 7-point operator with an odd number of rows above the diagonal-solve floor: the
+    operator solves on the diagonals by itself (two rows per lane, the last lane's second row padding) and
+    matches the oracle."""
+    from hpc_amd import device
+    A = workloads.stencil(workloads.STENCIL_3D7, 91, 81, 81, [0.5, 1.5, 2.5])
+    assert A.n % 2 == 1 and A.n >= 1 << 19
+    b = workloads.uniform(6, 1, A.n, -1, 1)
+    op = device.CsrOperator(_t(cuda, A.rowptr), _t(cuda, A.col), _t(cuda, A.val), rowptr_host=A.rowptr)
+    assert op.solve_layout == 4
+    x, info = op.solve(_t(cuda, A.diag), _t(cuda, b), 20000, 1e-10)
+    o = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, b, 20000, 1e-10)
+    _say(f"n={A.n} 7-point on diagonals: {info.executed} iterations, oracle {o['executed']}")
+    assert abs(info.executed - o["executed"]) <= 1
+    assert np.linalg.norm(x.cpu().numpy() - o["x"]) / np.linalg.norm(o["x"]) <= 1e-10
+
+
 def test_solve_layout_chosen_separately(cuda):
     """The solves keep their own creation-time layout; set_layout(L > 0) sets both, 0 restores both."""
     from hpc_amd import device
