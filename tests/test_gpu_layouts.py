@@ -357,6 +357,29 @@ def test_diagonal_solve_7point_above_floor(cuda):
     assert np.linalg.norm(x.cpu().numpy() - o["x"]) / np.linalg.norm(o["x"]) <= 1e-10
 
 
+def test_mixed_inner_on_diagonals_above_floor(cuda):
+    """Mixed precision on the same odd 7-point operator: the fp32-valued inner operator (split fp64
+    diagonal) solves on the diagonals, one row per lane; the refined solve reaches the tolerance in true
+    residual and agrees with the fp64 oracle's solution."""
+    from hpc_amd import device
+    A = workloads.stencil(workloads.STENCIL_3D7, 91, 81, 81, [0.5, 1.5, 2.5])
+    b = workloads.uniform(6, 1, A.n, -1, 1)
+    srp, scol, sv32, sdiag, exact = device.split_fp32(A.rowptr, A.col, A.val)
+    assert exact
+    inner = device.CsrOperator(_t(cuda, srp), _t(cuda, scol), _t(cuda, sv32), dsplit=_t(cuda, sdiag), rowptr_host=srp)
+    outer = device.CsrOperator(_t(cuda, A.rowptr), _t(cuda, A.col), _t(cuda, A.val), rowptr_host=A.rowptr)
+    assert inner.solve_layout == 4
+    xm, info = device.solve_mixed(outer, inner, _t(cuda, A.diag), _t(cuda, b), 1e-10)
+    xh = xm.cpu().numpy()
+    true = np.linalg.norm(b - A.to_scipy() @ xh) / np.linalg.norm(b)
+    o = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, b, 20000, 1e-10)
+    err = np.linalg.norm(xh - o["x"]) / np.linalg.norm(o["x"])
+    _say(f"mixed, inner on diagonals: outer {info.outer} inner {info.inner_total} (oracle {o['executed']}) "
+         f"true res {true:.2e} vs oracle {err:.2e}")
+    assert true < 1e-10
+    assert err < 1e-8
+
+
 def test_solve_layout_chosen_separately(cuda):
     """The solves keep their own creation-time layout; set_layout(L > 0) sets both, 0 restores both."""
     from hpc_amd import device
