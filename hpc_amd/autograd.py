@@ -7,13 +7,35 @@ Backward (A symmetric positive definite, so A^T = A):
     dL/dvalues[k]   = -lam[row_k] * x[col_k]  (SDDMM kernel on the sparsity pattern)
 Every stored entry is an independent parameter: the gradient of a symmetric
 pair (i,j), (j,i) is split between its two entries.
+A forward or adjoint solve that stops at max_iter with its recursive residual
+still at or above tol raises SolveNotConverged: a gradient from an
+unconverged adjoint would be silently wrong.
 """
 from __future__ import annotations
 
 import numpy as np
 import torch
 
+from ._lib import XjpcgError
 from .device import CsrOperator
+
+XJPCG_STATUS_EXECUTION_FAILED = 4
+
+
+class SolveNotConverged(XjpcgError):
+    """A solve inside the differentiable function ended above its tolerance (status EXECUTION_FAILED, as
+    pcg.h reports a divergent solve)."""
+
+    def __init__(self, which: str, info, tol: float, max_iter: int):
+        super().__init__(XJPCG_STATUS_EXECUTION_FAILED,
+                         f"{which} JPCG stopped after {info.executed} iterations (max_iter {max_iter}) at relative "
+                         f"residual {info.residual:.3e}, tolerance {tol:.3e}")
+        self.which, self.info = which, info
+
+
+def _converged(which: str, info, tol: float, max_iter: int):
+    if not (info.residual < tol):
+        raise SolveNotConverged(which, info, tol, max_iter)
 
 
 class SparsePattern:
@@ -64,6 +86,7 @@ class JPCGSolve(torch.autograd.Function):
         op = pattern.operator(values)
         diag = values[pattern.diag_index].contiguous()
         x, info = op.solve(diag, b, max_iter, tol)
+        _converged("forward", info, tol, max_iter)
         x = x.clone()
         ctx.pattern, ctx.tol, ctx.max_iter = pattern, tol, max_iter
         ctx.info = info
@@ -76,7 +99,8 @@ class JPCGSolve(torch.autograd.Function):
         pattern = ctx.pattern
         op = pattern.operator(values)
         diag = values[pattern.diag_index].contiguous()
-        lam, _ = op.solve(diag, gx.contiguous(), ctx.max_iter, ctx.tol)
+        lam, info = op.solve(diag, gx.contiguous(), ctx.max_iter, ctx.tol)
+        _converged("adjoint", info, ctx.tol, ctx.max_iter)
         lam = lam.clone()
         gvals = op.sddmm(lam, x) if ctx.needs_input_grad[0] else None
         gb = lam if ctx.needs_input_grad[1] else None

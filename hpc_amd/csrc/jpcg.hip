@@ -989,8 +989,15 @@ __global__ __launch_bounds__(kBlock) void sddmm_tiles(int ntiles, const int* __r
         }                                                      \
     } while (0)
 
-static int g_coop_grid[4] = {0, 0, 0, 0};  // cached co-resident grid per (value, column) type (one device)
+static int g_coop_grid[4] = {0, 0, 0, 0};  // cached persistent grid cap per (value, column) type (one device)
 
+// Largest grid the persistent kernel is launched with: one workgroup per CU
+// below the co-resident maximum.  A grid of exactly cus * per workgroups
+// needs every CU filled to its limit; the fp32-value instantiation (64
+// VGPRs, 8 waves per SIMD) then asks for every wave slot on the chip, and a
+// single slot held by any other dispatch leaves a workgroup unplaced while
+// the placed ones wait for it at the first grid barrier.  One workgroup of
+// slack per CU keeps every workgroup placeable.
 template <typename VT, typename CT>
 static hipError_t coop_grid(int* G) {
     const int slot = (sizeof(VT) == 4 ? 1 : 0) + (sizeof(CT) == 2 ? 2 : 0);
@@ -1006,7 +1013,7 @@ static hipError_t coop_grid(int* G) {
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, jpcg_persistent<VT, CT>, kBlock, 0);
     if (e != hipSuccess) return e;
     if (per < 1) return hipErrorCooperativeLaunchTooLarge;
-    g_coop_grid[slot] = cus * per;
+    g_coop_grid[slot] = cus * (per > 1 ? per - 1 : 1);
     *G = g_coop_grid[slot];
     return hipSuccess;
 }
@@ -1054,6 +1061,10 @@ hipError_t launch_cg_iteration(const CgParams& P, bool f32, bool c16, hipStream_
     } else {
         XJ_DISPATCH_VC(f32, c16, (cg_spmv<VT, CT><<<tile_groups(P.tiles), kBlock, 0, s>>>(P)));
     }
+    return launch_cg_vector_steps(P, s);
+}
+
+hipError_t launch_cg_vector_steps(const CgParams& P, hipStream_t s) {
     if (P.split_alpha) cg_alpha<<<P.nA2, kBlock, 0, s>>>(P);
     cg_update_r<<<P.G2, kBlock, 0, s>>>(P);
     cg_update_xp<<<P.G2, kBlock, 0, s>>>(P);

@@ -152,6 +152,9 @@ constexpr int kAlphaBlocks = 128;
 
 hipError_t launch_cg_setup(const CgParams& P, hipStream_t s);
 hipError_t launch_cg_iteration(const CgParams& P, bool f32, bool c16, hipStream_t s);
+// the steps of an iteration after its SpMV: the optional partial fold, the
+// residual update and the x / p update (for SpMV kernels launched elsewhere)
+hipError_t launch_cg_vector_steps(const CgParams& P, hipStream_t s);
 
 hipError_t persistent_grid(bool f32, bool c16, int* G);
 hipError_t launch_jpcg(const JpcgParams& P, bool f32, bool c16, int G, hipStream_t s);

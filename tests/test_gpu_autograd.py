@@ -120,3 +120,38 @@ def test_c5_finite_differences_64_entries(cuda):
     _say(f"FD relative errors: median {np.median(errs):.2e} max {errs.max():.2e}")
     assert np.median(errs) < 1e-7
     assert errs.max() < 1e-6
+
+
+def test_unconverged_forward_raises(cuda):
+    import torch
+    from hpc_amd import autograd
+    A, b, _ = workloads.c5_diffusion(m=32, length=4.0, seed=4)
+    pat, vals = _setup(cuda, A)
+    vals.requires_grad_(True)
+    bt = torch.from_numpy(b).to(cuda).requires_grad_(True)
+    with pytest.raises(autograd.SolveNotConverged) as ei:
+        autograd.jpcg_solve(pat, vals, bt, tol=1e-12, max_iter=5)
+    assert ei.value.which == "forward" and ei.value.status == 4
+    _say(str(ei.value))
+
+
+def test_unconverged_adjoint_raises(cuda):
+    """Forward right-hand side = an eigenvector of the 5-point Laplacian (constant diagonal, so Jacobi CG
+    converges in one iteration); the adjoint right-hand side x - x_target is not, and 5 iterations cannot
+    reach the tolerance: backward must raise instead of returning a gradient."""
+    import torch
+    from hpc_amd import autograd
+    m = 24
+    A = workloads.laplace2d(m)
+    s = np.sin(np.pi * np.arange(1, m + 1) / (m + 1))
+    b = np.outer(s, s).ravel()
+    pat, vals = _setup(cuda, A)
+    vals.requires_grad_(True)
+    bt = torch.from_numpy(b).to(cuda).requires_grad_(True)
+    x = autograd.jpcg_solve(pat, vals, bt, tol=1e-10, max_iter=5)
+    xt = torch.from_numpy(workloads.uniform(4, 2, A.n, -1.0, 1.0)).to(cuda)
+    loss = 0.5 * ((x - xt) ** 2).sum()
+    with pytest.raises(autograd.SolveNotConverged) as ei:
+        loss.backward()
+    assert ei.value.which == "adjoint"
+    _say(str(ei.value))
