@@ -65,6 +65,9 @@ _SIGNATURES = {
     "xjpcg_sddmm": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "xjpcg_solve_mixed": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_double, c_double,
                                   c_uint32, POINTER(c_uint32), POINTER(c_int64), POINTER(c_double), c_void_p]),
+    "xjpcg_csr_solve_static": (c_int, [c_int32, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                       c_uint32, c_double, POINTER(c_uint32), POINTER(c_double), POINTER(c_int64),
+                                       c_void_p]),
     "xjpcg_batch_last_error": (c_char_p, []),
     "xjpcg_batch_create": (c_int, [POINTER(c_void_p), c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
                                    c_void_p, c_int, c_void_p, c_void_p]),

@@ -40,6 +40,10 @@ xj::Operator& impl_of(xjpcg_op_t* op) {
 
 }  // namespace
 
+namespace xj {
+void set_dev_error(const std::string& m) { g_last_error = m; }
+}  // namespace xj
+
 extern "C" {
 
 const char* xjpcg_dev_last_error(void) { return g_last_error.c_str(); }

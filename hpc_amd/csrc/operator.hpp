@@ -28,6 +28,9 @@ inline Error exec_failed(const std::string& m) { return Error(XJPCG_STATUS_EXECU
 
 void hip_check(hipError_t e, const char* what);
 
+// message returned by xjpcg_dev_last_error() on the calling thread
+void set_dev_error(const std::string& m);
+
 // Owned device allocation.
 template <typename T>
 struct DevBuf {

@@ -203,6 +203,31 @@ def solve_mixed(outer: CsrOperator, inner: CsrOperator, diag: torch.Tensor, b: t
     return x, MixedInfo(no.value, it.value, res.value)
 
 
+STATIC_MAX_ROWS = 262144  # XJPCG_STATIC_MAX_ROWS
+
+
+def csr_solve_static(rowptr: torch.Tensor, col: torch.Tensor, val: torch.Tensor, diag: torch.Tensor,
+                     b: torch.Tensor, max_iter: int, tol: float,
+                     x: torch.Tensor | None = None) -> tuple[torch.Tensor, SolveInfo]:
+    """Allocation-free JPCG (xjpcg_csr_solve_static) on a device CSR with fp64 values and n <= STATIC_MAX_ROWS:
+    the library's work vectors live in its static per-GPU workspace; x (allocated here when not given) is the
+    only output."""
+    n = rowptr.numel() - 1
+    _dev_check(rowptr, torch.int32, "rowptr")
+    _dev_check(col, torch.int32, "col")
+    _dev_check(val, torch.float64, "val", col.numel())
+    _dev_check(diag, torch.float64, "diag", n)
+    _dev_check(b, torch.float64, "b", n)
+    if x is None:
+        x = torch.empty_like(b)
+    _dev_check(x, torch.float64, "x", n)
+    it, res, ex = ctypes.c_uint32(), ctypes.c_double(), ctypes.c_int64()
+    check_dev(lib().xjpcg_csr_solve_static(n, col.numel(), ptr(rowptr), ptr(col), ptr(val), ptr(diag), ptr(b), ptr(x),
+                                           int(max_iter), float(tol), ctypes.byref(it), ctypes.byref(res),
+                                           ctypes.byref(ex), _stream()))
+    return x, SolveInfo(it.value, ex.value, res.value)
+
+
 def split_fp32(rowptr_h, col_h, val_h):
     """Host helper: fp32 off-diagonal CSR + fp64 diagonal from an fp64 CSR (numpy arrays).
 

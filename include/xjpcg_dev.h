@@ -182,6 +182,28 @@ XJPCG_Status_t xjpcg_solve_mixed(xjpcg_op_t* outer, xjpcg_op_t* inner, const dou
                                  uint32_t inner_max_iter, uint32_t* h_outer, int64_t* h_inner_total,
                                  double* h_res, void* stream);
 
+/* Rows of the static workspace of xjpcg_csr_solve_static. */
+#define XJPCG_STATIC_MAX_ROWS 262144
+
+/* Allocation-free Jacobi PCG on the caller's device CSR (fp64 values, int32
+ * indices, the diagonal stored in the rows), with the semantics and outputs
+ * of xjpcg_solve: x0 = 0, at most maxIter+1 iterations, *h_iter =
+ * min(executed, maxIter), *h_res = ||r||/||b||, SUCCESS whether or not tol
+ * was met.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (serving loops, harnesses that forbid hipMalloc).
+ * It creates no operator, event, graph or device buffer: the work vectors
+ * (r, p, Ap, 1/diag), the partial sums and the loop state live in one
+ * static device array per GPU of 8.4 MB, sized for n <= XJPCG_STATIC_MAX_ROWS;
+ * larger n returns INVALID_VALUE.  The SpMV sums each row left to right on
+ * one lane (no row-tile plan), so iterates agree with xjpcg_solve on the
+ * same matrix within rounding of the p.Ap sum.  Synchronises `stream` every
+ * 16 iterations to read the stop flag.  Calls that overlap on one GPU share
+ * the array and race; the last call's contents stay in it. */
+XJPCG_Status_t xjpcg_csr_solve_static(int32_t n, int64_t nnz, const int32_t* d_rowptr, const int32_t* d_col,
+                                      const double* d_val, const double* d_diag, const double* d_b, double* d_x,
+                                      uint32_t max_iter, double tol, uint32_t* h_iter, double* h_res,
+                                      int64_t* h_executed, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
