@@ -45,6 +45,7 @@ _SIGNATURES = {
     "xjpcg_op_set_values": (c_int, [c_void_p, c_void_p]),
     "xjpcg_op_set_compact": (c_int, [c_void_p, c_int, POINTER(c_int)]),
     "xjpcg_op_info": (c_int, [c_void_p, POINTER(c_int32), POINTER(c_int64), POINTER(c_int32), POINTER(c_int32)]),
+    "xjpcg_op_spmv_bytes": (c_int, [c_void_p, c_int, POINTER(c_int64)]),
     "xjpcg_op_set_path": (c_int, [c_void_p, c_int, POINTER(c_int)]),
     "xjpcg_op_set_layout": (c_int, [c_void_p, c_int, c_void_p, POINTER(c_int)]),
     "xjpcg_op_solve_layout": (c_int, [c_void_p, POINTER(c_int)]),

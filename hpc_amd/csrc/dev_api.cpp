@@ -146,6 +146,14 @@ XJPCG_Status_t xjpcg_op_solve_layout(const xjpcg_op_t* op, int* h_layout) {
     });
 }
 
+XJPCG_Status_t xjpcg_op_spmv_bytes(const xjpcg_op_t* op, int layout, int64_t* h_bytes) {
+    return guarded([&] {
+        const xj::Operator& o = impl_of(const_cast<xjpcg_op_t*>(op));
+        if (layout < 0 || layout > 4) throw xj::invalid("layout must be 0 (in use), 1, 2, 3 or 4.");
+        if (h_bytes) *h_bytes = o.spmv_bytes(layout);
+    });
+}
+
 XJPCG_Status_t xjpcg_op_info(const xjpcg_op_t* op, int32_t* n, int64_t* nnz, int32_t* ntiles, int32_t* grid) {
     return guarded([&] {
         const xj::Operator& o = impl_of(const_cast<xjpcg_op_t*>(op));

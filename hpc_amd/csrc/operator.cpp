@@ -531,6 +531,34 @@ float Operator::time_layout(int layout, hipStream_t s) {
     return ms;
 }
 
+int64_t Operator::spmv_bytes(int layout) const {
+    const int L = layout == kLayoutAuto ? this->layout() : layout;
+    const int64_t n = A_.n, nnz = A_.nnz;
+    const int64_t vb = A_.val_f32 ? 4 : 8;
+    const int64_t vec = 16 * n + (A_.dsplit ? 8 * n : 0);  // x and y once, the split diagonal once
+    switch (L) {
+        case kTiles: {
+            // values, the column stream in use, both row pointers of each row
+            const int64_t cb = A_.col16 ? 2 : 4;
+            return (vb + cb) * nnz + 4 * (n + 1) + vec;
+        }
+        case kPatterns:
+            if (!has_pat_) return -1;
+            // values, row pointers, one mask per row, 32 offsets per tile
+            return vb * nnz + 4 * (n + 1) + 4 * n + 128 * (int64_t)A_.ntiles + vec;
+        case kDiagonals:
+            if (!has_dia_) return -1;
+            // every diagonal's zero-padded value array, one mask per row
+            return vb * (int64_t)A_.dia.ndia * A_.dia.ld + 4 * (int64_t)A_.dia.ld + vec;
+        case kPanels:
+            if (!has_pan_) return -1;
+            // value, uint16 column and uint16 row offset per entry, the segment table
+            return (vb + 4) * nnz + 4 * ((int64_t)A_.pan.nb * A_.pan.P + 1) + vec;
+        default:
+            return -1;
+    }
+}
+
 void Operator::set_layout(int layout, hipStream_t s) {
     int want = layout == kLayoutAuto ? auto_layout_ : layout;
     if (want != kTiles && want != kPanels && want != kPatterns && want != kDiagonals)

@@ -138,6 +138,10 @@ class Operator {
         return A_.pan_on ? kPanels : A_.pat_on ? kPatterns : A_.dia_on ? kDiagonals : kTiles;
     }
     int solve_layout() const { return solve_layout_; }
+    // Bytes one standalone SpMV launch on `layout` (0: the one in use) must
+    // read and write: the layout's matrix streams (zero padding included),
+    // its per-row tables, x once and y once; -1 when the layout is not built.
+    int64_t spmv_bytes(int layout) const;
     // use (or stop using) the int16 column stream when it exists
     void set_compact(bool on);
 

@@ -90,6 +90,13 @@ class CsrOperator:
                                       ctypes.byref(g)))
         return {"n": n.value, "nnz": nnz.value, "ntiles": nt.value, "grid": g.value}
 
+    def spmv_bytes(self, layout: int = 0) -> int:
+        """Bytes one SpMV launch on `layout` (0: the layout in use) moves: its matrix streams with padding, its
+        per-row tables, x once and y once (-1 when that layout is not built)."""
+        nb = ctypes.c_int64()
+        check_dev(lib().xjpcg_op_spmv_bytes(self._h, int(layout), ctypes.byref(nb)))
+        return nb.value
+
     def set_values(self, val: torch.Tensor):
         """Point the operator at new values (same dtype and length); pattern and tiles are kept."""
         if val.dtype != self.val.dtype:

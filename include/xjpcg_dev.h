@@ -106,6 +106,14 @@ XJPCG_Status_t xjpcg_pattern_plan(int32_t n, const int32_t* h_rowptr, const int3
                                   int32_t* offs, int* h_ok);
 /* n, nnz, number of row tiles, grid of the persistent kernel */
 XJPCG_Status_t xjpcg_op_info(const xjpcg_op_t* op, int32_t* n, int64_t* nnz, int32_t* ntiles, int32_t* grid);
+/* Bytes one SpMV launch on `layout` (0 = the layout in use, 1-4 as for
+ * xjpcg_op_set_layout) must move: the layout's matrix streams including its
+ * zero padding (diagonals), its per-row tables (row pointers, masks,
+ * segment table), x read once, y written once and the split diagonal read
+ * once.  *h_bytes = -1 when that layout has not been built.  Divided by a
+ * launch's duration it gives the SpMV's HBM rate on the bytes its layout
+ * needs (no counterpart in the reference). */
+XJPCG_Status_t xjpcg_op_spmv_bytes(const xjpcg_op_t* op, int layout, int64_t* h_bytes);
 
 /* Loop used by xjpcg_solve on this operator: 0 = automatic (the faster
  * measured loop), 1 = one persistent cooperative launch per solve with grid
