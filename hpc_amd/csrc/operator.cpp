@@ -242,9 +242,8 @@ Operator::Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t*
     // (stencils), the diagonal layout when the whole matrix has at most 32
     // (and its value arrays stay within twice the stored entries), and, when
     // the columns are too spread for the int16 stream and x is far larger than
-    // an XCD's L2, the panel layout (only when at least 15% faster than the
-    // tiles).  Every layout sums each row left to right, so the SpMV result
-    // does not depend on the choice.
+    // an XCD's L2, the panel layout.  Every layout sums each row left to
+    // right, so the SpMV result does not depend on the choice.
     //
     // The solves' layout follows from the matrix instead, so that operators
     // built from the same matrix give bit-identical iterates whatever the
@@ -252,10 +251,12 @@ Operator::Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t*
     // matrix has at least kDiaSolveRows rows (below that the fp64 kernel's
     // 512-row workgroups leave the chip half empty: a 512^2 diffusion solve
     // took 61.9 ms on them against 56.8 ms on the tiles), else
-    // the panels when accepted, else the tiles or the pattern tiles (whichever
-    // SpMV is faster; both give the same per-tile p.Ap partials, so the same
-    // iterates).  The diagonal and panel layouts group the p.Ap partials their
-    // own way (per wavefront, per row block).
+    // the panels when their rule holds (no int16 column stream, at least
+    // kPanelAutoRows rows, columns ascending in panel order), else the tiles
+    // or the pattern tiles (whichever SpMV is faster; both give the same
+    // per-tile p.Ap partials, so the same iterates).  The diagonal and panel
+    // layouts group the p.Ap partials their own way (per wavefront, per row
+    // block).
     if (nnz > 0) {
         const float t_tiles = time_layout(kTiles, s);
         float best = t_tiles, t_pat = 0.0f;
@@ -273,15 +274,17 @@ Operator::Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t*
                 auto_layout_ = kDiagonals;
             }
         }
+        // panels by rule (columns too spread for the int16 stream, x beyond
+        // an XCD's L2, every row's columns in ascending panel order), so the
+        // solve layout does not depend on a timing; the standalone SpMV still
+        // takes them only when they time faster
         bool panels_ok = false;
         if (!has_col16_ && n >= kPanelAutoRows && build_panels(s)) {
+            panels_ok = true;
             const float t_pan = time_layout(kPanels, s);
-            if (t_pan > 0.0f && t_pan < 0.85f * t_tiles) {
-                panels_ok = true;
-                if (t_pan < best) {
-                    best = t_pan;
-                    auto_layout_ = kPanels;
-                }
+            if (t_pan > 0.0f && t_pan < best) {
+                best = t_pan;
+                auto_layout_ = kPanels;
             }
         }
         auto_solve_ = (t_pat > 0.0f && t_pat < t_tiles) ? kPatterns : kTiles;

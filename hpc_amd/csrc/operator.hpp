@@ -125,8 +125,8 @@ class Operator {
     }
     // SpMV layout: kTiles (row tiles over the CSR arrays) or kPanels (the
     // panel-swept copy, for columns scattered over x).  kLayoutAuto restores
-    // the choice made at creation: panels when they were measured at least
-    // 15% faster on an operator whose columns do not fit the int16 stream.
+    // the choice made at creation: the fastest layout measured (panels are
+    // built when the columns do not fit the int16 stream).
     // kPatterns runs the row tiles on column-offset patterns (stencil
     // operators; bit-identical to kTiles), chosen at creation when faster.
     // kDiagonals runs on one value array per global column offset (stencils

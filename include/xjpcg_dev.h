@@ -70,7 +70,8 @@ XJPCG_Status_t xjpcg_op_set_values(xjpcg_op_t* op, const void* d_val);
  * a row has no entry, a 32-bit presence mask per row; each row's entries are
  * summed in column order, two rows per lane for fp64 values: same sums as 1).
  * At creation the operator times the layouts that apply and keeps the fastest
- * (panels only when at least 15% faster than the tiles); layout = 0 restores
+ * (panels are built for operators of at least 2^18 rows whose columns do not
+ * fit the int16 stream); layout = 0 restores
  * that choice, layout < 0 only queries.  Layout 2 needs each row's columns in
  * ascending order; layout 3 needs strictly ascending columns, rows of at most
  * 32 entries and at most 32 offsets per tile; layout 4 strictly ascending
@@ -82,7 +83,7 @@ XJPCG_Status_t xjpcg_op_set_layout(xjpcg_op_t* op, int layout, void* stream, int
  * timing, so that operators built from the same matrix give bit-identical
  * iterates: diagonals when they apply with at most 25% padding slots and
  * the operator has at least 2^19 rows, else
- * the panels when the SpMV kept them, else the tiles or the pattern tiles
+ * the panels when they were built (the rule above), else the tiles or the pattern tiles
  * (the faster SpMV of the two; both give the same iterates).  It can differ
  * from the standalone SpMV's layout.  xjpcg_op_set_layout with layout > 0
  * sets both; layout = 0 restores both creation-time choices. */
