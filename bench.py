@@ -367,11 +367,23 @@ def bench_batched(args):
                                        steps, warm, stream)
     log(f"mixed batch: {np.median(msm):.1f} ms, outer {steps_m}, launched {minfo.launched}, max true res "
         f"{float(minfo.residual.max()):.2e}")
+    # the inner batch with fp32 work vectors too (fp64 arithmetic, dots and outer loop)
+    inner32 = batched.Batch.from_host(asm32, R, dev)
+    inner32.set_vectors(32)
+    if args.tune:
+        inner32.configure(tune=True)
+    xw = torch.empty_like(b)
+    msw, (xw, winfo, steps_w) = _timed(lambda: batched.solve_mixed(outer, inner32, diag, b, tol,
+                                                                   inner_tol=args.inner_tol, x=xw),
+                                       steps, warm, stream)
+    log(f"mixed batch, fp32 work vectors: {np.median(msw):.1f} ms, outer {steps_w}, launched {winfo.launched}, max "
+        f"true res {float(winfo.residual.max()):.2e}")
+    del inner32
     sizes = np.diff(asm64["sys_row"])
     order = np.argsort(sizes, kind="stable")
     picks = [int(order[-1]), int(order[0]), int(order[len(order) // 2])]
     th, _ = cpu_threads()
-    xh, xmh = x.cpu().numpy(), xm.cpu().numpy()
+    xh, xmh, xwh = x.cpu().numpy(), xm.cpu().numpy(), xw.cpu().numpy()
     checks = []
     for s in picks:
         r0, r1 = int(asm64["sys_row"][s]), int(asm64["sys_row"][s + 1])
@@ -381,7 +393,8 @@ def bench_batched(args):
         checks.append({"system": s, "n": A.n, "rhs": 0, "oracle_iterations": ref["executed"],
                        "iterations": int(its[s, 0]),
                        "fp64_rel_err": float(np.linalg.norm(xh[r0:r1, 0] - ref["x"]) / nr),
-                       "mixed_rel_err": float(np.linalg.norm(xmh[r0:r1, 0] - ref["x"]) / nr)})
+                       "mixed_rel_err": float(np.linalg.norm(xmh[r0:r1, 0] - ref["x"]) / nr),
+                       "mixed_fp32_vectors_rel_err": float(np.linalg.norm(xwh[r0:r1, 0] - ref["x"]) / nr)})
     # the batch runs a system while any of its pairs runs: its matrix moves max_j(its) times
     nb = batch_bytes(mats, its, its.max(axis=1))
     t64 = float(np.median(ms64))
@@ -397,6 +410,10 @@ def bench_batched(args):
                      "inner_tol": args.inner_tol, "max_true_res": float(minfo.residual.max()),
                      "config": inner.info()},
            "speedup_mixed_vs_fp64": round(t64 / float(np.median(msm)), 3),
+           "mixed_fp32_vectors": {"ms": round(float(np.median(msw)), 2), "outer": steps_w,
+                                  "launched_iterations": winfo.launched,
+                                  "max_true_res": float(winfo.residual.max()),
+                                  "speedup_vs_fp64": round(t64 / float(np.median(msw)), 3)},
            "bytes_basis": "per system and batched iteration it runs: 12 nnz + 4(n+1); per (system, rhs) "
                           "iteration: 112 n (the reference's SpMV / x / r-z / p stage split)",
            "checks": checks}

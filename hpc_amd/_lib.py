@@ -77,6 +77,7 @@ _SIGNATURES = {
                                  POINTER(c_int32)]),
     "xjpcg_batch_set_config": (c_int, [c_void_p, c_int32, c_int32, c_int32]),
     "xjpcg_batch_set_layout": (c_int, [c_void_p, c_int32, POINTER(c_int32), c_void_p]),
+    "xjpcg_batch_set_vectors": (c_int, [c_void_p, c_int32, POINTER(c_int32)]),
     "xjpcg_batch_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_double, c_void_p,
                                   c_void_p, POINTER(c_int64), c_void_p]),
     "xjpcg_batch_solve_mixed": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_double,

@@ -131,6 +131,19 @@ class Batch:
         _check(lib().xjpcg_batch_set_layout(self._h, -1, ctypes.byref(act), None))
         return act.value
 
+    def set_vectors(self, bits: int) -> int:
+        """Work vectors r, p, Ap of the solve in fp64 (bits=64) or fp32 (bits=32; x, arithmetic and dot products
+        stay fp64; for the inner batch of solve_mixed); returns the setting."""
+        act = ctypes.c_int32()
+        _check(lib().xjpcg_batch_set_vectors(self._h, int(bits), ctypes.byref(act)))
+        return act.value
+
+    @property
+    def vector_bits(self) -> int:
+        act = ctypes.c_int32()
+        _check(lib().xjpcg_batch_set_vectors(self._h, 0, ctypes.byref(act)))
+        return act.value
+
     def configure(self, tile_rows: int = 0, block: int = 0, tune: bool = False):
         _check(lib().xjpcg_batch_set_config(self._h, int(tile_rows), int(block), 1 if tune else 0))
 

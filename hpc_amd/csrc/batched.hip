@@ -57,24 +57,39 @@ __device__ __forceinline__ double system_sum(const double* part, int t0, int t1,
     return group_sum<R>(s, red);
 }
 
-template <int R>
+// Work vectors r, p, Ap are stored as WT: double, or float for the inner
+// batch of a refinement (WT = float cuts their traffic in half).  Every
+// value is widened to double before any arithmetic and rounded to WT only
+// when stored; dot products use the stored (rounded) values and accumulate
+// in double.  x stays double: rounding x would leave a true residual of
+// about eps32 * cond(A) whatever the recursive one (measured 3.5e-4 on the
+// 512-system batch), while rounding r, p and Ap only perturbs the recursion
+// at the size of the current residual.  With WT = double every expression
+// below is the plain fp64 loop.
+template <typename WT>
+__device__ __forceinline__ WT* wvec(double* p) { return reinterpret_cast<WT*>(p); }
+
+template <int R, typename WT>
 __global__ __launch_bounds__(kMaxBlock) void bsetup(BatchParams P) {
     __shared__ double red[kMaxBlock];
     const int t = (int)blockIdx.x;
     const int j = threadIdx.x % R, g = threadIdx.x / R, G = blockDim.x / R;
     const int r0 = P.tile_row[t], r1 = P.tile_row[t + 1];
+    WT* __restrict__ r = wvec<WT>(P.r);
+    WT* __restrict__ p = wvec<WT>(P.p);
     double dot = 0.0, rz = 0.0;
     for (int i = r0 + g; i < r1; i += G) {
         const size_t e = (size_t)i * R + j;
         const double bi = P.b[e];
+        const double ri = (double)(WT)bi;
         const double Ji = 1.0 / P.diag[i];
-        const double zi = Ji * bi;
+        const double zi = Ji * ri;
         if (j == 0) P.J[i] = Ji;
-        P.r[e] = bi;
-        P.p[e] = zi;
+        r[e] = (WT)ri;
+        p[e] = (WT)zi;
         P.x[e] = 0.0;
-        dot += bi * bi;
-        rz += bi * zi;
+        dot += bi * bi;  // the stop rule's ||b|| from the exact right-hand side
+        rz += ri * zi;
     }
     dot = group_sum<R>(dot, red);
     rz = group_sum<R>(rz, red);
@@ -133,15 +148,15 @@ struct SpmmLanes {
     static constexpr int L = R / Q;
 };
 
-template <int Q>
-__device__ __forceinline__ void load_q(const double* __restrict__ p, double (&out)[Q]) {
+template <int Q, typename T>
+__device__ __forceinline__ void load_q(const T* __restrict__ p, double (&out)[Q]) {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) out[q] = p[q];
+    for (int q = 0; q < Q; ++q) out[q] = (double)p[q];
 }
 
 // acc[q] = row i of A times column (q0 + q) of the [N][R] block xin
-template <typename VT, int R>
-__device__ __forceinline__ void brow(const BatchParams& P, const VT* __restrict__ val, const double* __restrict__ xin,
+template <typename VT, int R, typename XT>
+__device__ __forceinline__ void brow(const BatchParams& P, const VT* __restrict__ val, const XT* __restrict__ xin,
                                      int i, int q0, double (&acc)[SpmmLanes<R>::Q]) {
     constexpr int Q = SpmmLanes<R>::Q;
     const int k0 = P.rowptr[i], k1 = P.rowptr[i + 1];
@@ -181,9 +196,9 @@ __device__ __forceinline__ void brow(const BatchParams& P, const VT* __restrict_
 // system for offsets the row lacks) load with its mask, no index first; the
 // entries are added in ascending offset order -- the CSR order of the row,
 // so the sums are those of brow.
-template <typename VT, int R>
+template <typename VT, int R, typename XT>
 __device__ __forceinline__ void brow_dia(const BatchParams& P, const BDiaSys& Y, const VT* __restrict__ dval,
-                                         const double* __restrict__ xin, int i, int q0,
+                                         const XT* __restrict__ xin, int i, int q0,
                                          double (&acc)[SpmmLanes<R>::Q]) {
     constexpr int Q = SpmmLanes<R>::Q;
     const uint32_t m = P.dmask[i];
@@ -225,7 +240,7 @@ __device__ __forceinline__ void brow_dia(const BatchParams& P, const BDiaSys& Y,
     }
 }
 
-template <typename VT, int R, bool DIA>
+template <typename VT, int R, bool DIA, typename WT>
 __global__ __launch_bounds__(kMaxBlock) void bspmm_dot(BatchParams P) {
     constexpr int Q = SpmmLanes<R>::Q, L = SpmmLanes<R>::L;
     __shared__ double red[kMaxBlock * Q];
@@ -238,22 +253,24 @@ __global__ __launch_bounds__(kMaxBlock) void bspmm_dot(BatchParams P) {
     const int q0 = l * Q;
     const int r0 = P.tile_row[t], r1 = P.tile_row[t + 1];
     const VT* __restrict__ val = static_cast<const VT*>(P.val);
-    const double* __restrict__ p = P.p;
+    const WT* __restrict__ p = wvec<WT>(P.p);
+    WT* __restrict__ Ap = wvec<WT>(P.Ap);
     double dl[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) dl[q] = 0.0;
     for (int i = r0 + g; i < r1; i += G) {
         double acc[Q];
         if (DIA)
-            brow_dia<VT, R>(P, Y, static_cast<const VT*>(P.dval), p, i, q0, acc);
+            brow_dia<VT, R, WT>(P, Y, static_cast<const VT*>(P.dval), p, i, q0, acc);
         else
-            brow<VT, R>(P, val, p, i, q0, acc);
+            brow<VT, R, WT>(P, val, p, i, q0, acc);
         double pi[Q];
         load_q<Q>(p + (size_t)i * R + q0, pi);
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            P.Ap[(size_t)i * R + q0 + q] = acc[q];
-            dl[q] += pi[q] * acc[q];
+            const WT a = (WT)acc[q];
+            Ap[(size_t)i * R + q0 + q] = a;
+            dl[q] += pi[q] * (double)a;
         }
     }
     // per right-hand side: sum over the row groups in fixed order g = 0, 1, ...
@@ -312,7 +329,7 @@ __global__ __launch_bounds__(kMaxBlock) void bsys_alpha(BatchParams P) {
     if (s == 0 && threadIdx.x == 0) *P.any_active = 0;
 }
 
-template <int R>
+template <int R, typename WT>
 __global__ __launch_bounds__(kMaxBlock) void bupdate_r(BatchParams P) {
     __shared__ double red[kMaxBlock];
     const int t = (int)blockIdx.x;
@@ -321,11 +338,13 @@ __global__ __launch_bounds__(kMaxBlock) void bupdate_r(BatchParams P) {
     const int j = threadIdx.x % R, g = threadIdx.x / R, G = blockDim.x / R;
     const int r0 = P.tile_row[t], r1 = P.tile_row[t + 1];
     const double alpha = P.alpha[s * R + j];
+    WT* __restrict__ r = wvec<WT>(P.r);
+    const WT* __restrict__ Ap = wvec<WT>(P.Ap);
     double rr = 0.0, rz = 0.0;
     for (int i = r0 + g; i < r1; i += G) {
         const size_t e = (size_t)i * R + j;
-        const double ri = -alpha * P.Ap[e] + P.r[e];
-        P.r[e] = ri;
+        const double ri = (double)(WT)(-alpha * (double)Ap[e] + (double)r[e]);
+        r[e] = (WT)ri;
         const double zi = P.J[i] * ri;
         rr += ri * ri;
         rz += ri * zi;
@@ -389,7 +408,7 @@ __global__ __launch_bounds__(kMaxBlock) void bsys_beta(BatchParams P) {
     }
 }
 
-template <int R>
+template <int R, typename WT>
 __global__ __launch_bounds__(kMaxBlock) void bupdate_xp(BatchParams P) {
     const int t = (int)blockIdx.x;
     const int s = P.tile_sys[t];
@@ -399,11 +418,13 @@ __global__ __launch_bounds__(kMaxBlock) void bupdate_xp(BatchParams P) {
     if (!P.upd[q]) return;
     const int r0 = P.tile_row[t], r1 = P.tile_row[t + 1];
     const double alpha = P.alpha[q], beta = P.beta[q];
+    WT* __restrict__ p = wvec<WT>(P.p);
+    const WT* __restrict__ r = wvec<WT>(P.r);
     for (int i = r0 + g; i < r1; i += G) {
         const size_t e = (size_t)i * R + j;
-        const double pi = P.p[e];
+        const double pi = (double)p[e];
         P.x[e] = alpha * pi + P.x[e];
-        P.p[e] = beta * pi + P.J[i] * P.r[e];
+        p[e] = (WT)(beta * pi + P.J[i] * (double)r[e]);
     }
 }
 
@@ -460,28 +481,39 @@ __global__ void baxpy(size_t m, const double* __restrict__ d, double* __restrict
 // --------------------------------------------------------------------------
 // launchers
 
-template <int R>
-static void iteration_r(const BatchParams& P, bool f32, int block, hipStream_t s) {
+template <int R, typename WT>
+static void iteration_rw(const BatchParams& P, bool f32, int block, hipStream_t s) {
     if (P.dia_on) {
         if (f32)
-            bspmm_dot<float, R, true><<<P.T, block, 0, s>>>(P);
+            bspmm_dot<float, R, true, WT><<<P.T, block, 0, s>>>(P);
         else
-            bspmm_dot<double, R, true><<<P.T, block, 0, s>>>(P);
+            bspmm_dot<double, R, true, WT><<<P.T, block, 0, s>>>(P);
     } else {
         if (f32)
-            bspmm_dot<float, R, false><<<P.T, block, 0, s>>>(P);
+            bspmm_dot<float, R, false, WT><<<P.T, block, 0, s>>>(P);
         else
-            bspmm_dot<double, R, false><<<P.T, block, 0, s>>>(P);
+            bspmm_dot<double, R, false, WT><<<P.T, block, 0, s>>>(P);
     }
     bsys_alpha<R><<<P.S, kMaxBlock, 0, s>>>(P);
-    bupdate_r<R><<<P.T, block, 0, s>>>(P);
+    bupdate_r<R, WT><<<P.T, block, 0, s>>>(P);
     bsys_beta<R><<<P.S, kMaxBlock, 0, s>>>(P);
-    bupdate_xp<R><<<P.T, block, 0, s>>>(P);
+    bupdate_xp<R, WT><<<P.T, block, 0, s>>>(P);
 }
 
 template <int R>
-static void setup_r(const BatchParams& P, int block, hipStream_t s) {
-    bsetup<R><<<P.T, block, 0, s>>>(P);
+static void iteration_r(const BatchParams& P, bool f32, bool w32, int block, hipStream_t s) {
+    if (w32)
+        iteration_rw<R, float>(P, f32, block, s);
+    else
+        iteration_rw<R, double>(P, f32, block, s);
+}
+
+template <int R>
+static void setup_r(const BatchParams& P, bool w32, int block, hipStream_t s) {
+    if (w32)
+        bsetup<R, float><<<P.T, block, 0, s>>>(P);
+    else
+        bsetup<R, double><<<P.T, block, 0, s>>>(P);
     bsys_setup<R><<<P.S, kMaxBlock, 0, s>>>(P);
 }
 
@@ -523,15 +555,16 @@ static void mask_r(const BatchParams& P, int block, const int* keep, double* v, 
         default: return hipErrorInvalidValue;         \
     }
 
-hipError_t batch_launch_setup(const BatchParams& P, int R, int block, hipStream_t s) {
-    XJ_DISPATCH_R(R, setup_r<RR>(P, block, s));
+hipError_t batch_launch_setup(const BatchParams& P, int R, bool w32, int block, hipStream_t s) {
+    XJ_DISPATCH_R(R, setup_r<RR>(P, w32, block, s));
     return hipGetLastError();
 }
 
-hipError_t batch_launch_iteration(const BatchParams& P, int R, bool f32, int block, hipStream_t s) {
-    XJ_DISPATCH_R(R, iteration_r<RR>(P, f32, block, s));
+hipError_t batch_launch_iteration(const BatchParams& P, int R, bool f32, bool w32, int block, hipStream_t s) {
+    XJ_DISPATCH_R(R, iteration_r<RR>(P, f32, w32, block, s));
     return hipGetLastError();
 }
+
 
 hipError_t batch_launch_spmm(const BatchParams& P, int R, bool f32, int block, const double* x, double* y,
                              const double* b, hipStream_t s) {

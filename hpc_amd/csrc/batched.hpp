@@ -46,8 +46,10 @@ struct BatchParams {
     uint32_t max_rounds = 0;
 };
 
-hipError_t batch_launch_setup(const BatchParams& P, int R, int block, hipStream_t s);
-hipError_t batch_launch_iteration(const BatchParams& P, int R, bool f32, int block, hipStream_t s);
+// w32: the work vectors r, p, Ap (P.r, P.p, P.Ap) hold fp32 values (x,
+// arithmetic and dot products stay fp64)
+hipError_t batch_launch_setup(const BatchParams& P, int R, bool w32, int block, hipStream_t s);
+hipError_t batch_launch_iteration(const BatchParams& P, int R, bool f32, bool w32, int block, hipStream_t s);
 hipError_t batch_launch_spmm(const BatchParams& P, int R, bool f32, int block, const double* x, double* y,
                              const double* b, hipStream_t s);
 hipError_t batch_launch_norms(const BatchParams& P, int R, int block, const double* v, double* part, double* out,

@@ -168,14 +168,18 @@ class Batch {
     float time_iteration(double* v, bool busy) {
         double* x = v;
         double* d = v + (size_t)N_ * R_;
-        hip_check(launch_fill((int64_t)N_ * R_, 1.0, r_.ptr, work_), "tune input");
-        BatchParams P = params(d, r_.ptr, x, busy ? 0.0 : INFINITY, 0xFFFFFFF0u);
+        // b = 1 lives in r_ for fp64 work vectors (the set-up reads b[e] before
+        // it writes r[e] = b[e]); fp32 work vectors are narrower than b, so b
+        // then lives in Ap_, which the set-up does not touch
+        double* bsrc = w32_ ? Ap_.ptr : r_.ptr;
+        hip_check(launch_fill((int64_t)N_ * R_, 1.0, bsrc, work_), "tune input");
+        BatchParams P = params(d, bsrc, x, busy ? 0.0 : INFINITY, 0xFFFFFFF0u);
         hip_check(hipMemsetAsync(any_.ptr, 0, sizeof(int), work_), "reset flag");
-        hip_check(batch_launch_setup(P, R_, block_, work_), "tune set-up");
+        hip_check(batch_launch_setup(P, R_, w32_, block_, work_), "tune set-up");
         EventPair ev;
-        for (int w = 0; w < 2; ++w) hip_check(batch_launch_iteration(P, R_, f32_, block_, work_), "tune");
+        for (int w = 0; w < 2; ++w) hip_check(batch_launch_iteration(P, R_, f32_, w32_, block_, work_), "tune");
         hip_check(hipEventRecord(ev.e0, work_), "event");
-        for (int w = 0; w < 5; ++w) hip_check(batch_launch_iteration(P, R_, f32_, block_, work_), "tune");
+        for (int w = 0; w < 5; ++w) hip_check(batch_launch_iteration(P, R_, f32_, w32_, block_, work_), "tune");
         hip_check(hipEventRecord(ev.e1, work_), "event");
         hip_check(hipEventSynchronize(ev.e1), "event");
         return ev.ms() / 5.0f;
@@ -254,7 +258,7 @@ class Batch {
         sync_user(user);
         BatchParams P = params(diag, b, x, tol, rounds);
         hip_check(hipMemsetAsync(any_.ptr, 0, sizeof(int), work_), "reset flag");
-        hip_check(batch_launch_setup(P, R_, block_, work_), "batch set-up");
+        hip_check(batch_launch_setup(P, R_, w32_, block_, work_), "batch set-up");
         // iterations run as replays of a captured graph of kChunk iterations
         if (!gexec_ || g_x_ != x || g_rounds_ != rounds) build_graph(P);
         int64_t launched = 0;
@@ -267,7 +271,7 @@ class Batch {
                 launched += kChunk;
             } else {
                 for (uint32_t c = 0; c < left; ++c)
-                    hip_check(batch_launch_iteration(P, R_, f32_, block_, work_), "batch iteration");
+                    hip_check(batch_launch_iteration(P, R_, f32_, w32_, block_, work_), "batch iteration");
                 launched += left;
             }
             hip_check(hipMemcpyAsync(&host_flags_[slot], any_.ptr, sizeof(int), hipMemcpyDeviceToHost, work_),
@@ -285,6 +289,18 @@ class Batch {
         hip_check(hipStreamSynchronize(work_), "batch solve");
         return launched;
     }
+
+    // 64: work vectors in fp64 (the default); 32: r, p and Ap stored in fp32
+    // with x, the arithmetic and the dot products in fp64 (the inner batch of
+    // a refinement, where they cut the vector traffic by about 40%)
+    void set_vector_bits(int bits) {
+        if (bits != 32 && bits != 64) throw invalid("vector bits must be 32 or 64.");
+        if ((bits == 32) != w32_) {
+            w32_ = bits == 32;
+            drop_graph();
+        }
+    }
+    int vector_bits() const { return w32_ ? 32 : 64; }
 
     // y = A x or y = b - A x on the caller's stream order
     void spmm(const double* x, double* y, const double* b, hipStream_t user) {
@@ -443,7 +459,7 @@ class Batch {
         hipGraph_t g = nullptr;
         hip_check(hipStreamBeginCapture(work_, hipStreamCaptureModeThreadLocal), "begin capture");
         for (int c = 0; c < kChunk; ++c) {
-            hipError_t e = batch_launch_iteration(P, R_, f32_, block_, work_);
+            hipError_t e = batch_launch_iteration(P, R_, f32_, w32_, block_, work_);
             if (e != hipSuccess) {
                 (void)hipStreamEndCapture(work_, &g);
                 if (g) (void)hipGraphDestroy(g);
@@ -478,6 +494,7 @@ class Batch {
 
     int N_ = 0, S_ = 0, R_ = 0, T_ = 0, nnz_ = 0, tile_rows_ = 0, block_ = 256;
     bool f32_ = false;
+    bool w32_ = false;               // fp32 work vectors r, p, Ap
     std::vector<int32_t> sys_row_;
     BatchParams P_;
     hipStream_t work_ = nullptr;
@@ -507,6 +524,9 @@ struct xjpcg_batch {
 
 namespace {
 thread_local std::string g_batch_error;
+
+// smallest relative tolerance a refinement step asks of an inner batch with fp32 work vectors
+constexpr double kW32StepTol = 1e-7;
 
 template <typename F>
 XJPCG_Status_t bguard(F f) {
@@ -592,6 +612,14 @@ extern "C" XJPCG_Status_t xjpcg_batch_solve(xjpcg_batch_t* batch, const double* 
     });
 }
 
+extern "C" XJPCG_Status_t xjpcg_batch_set_vectors(xjpcg_batch_t* batch, int32_t bits, int32_t* h_bits) {
+    return bguard([&] {
+        xj::Batch& B = impl_of(batch);
+        if (bits > 0) B.set_vector_bits(bits);
+        if (h_bits) *h_bits = B.vector_bits();
+    });
+}
+
 extern "C" XJPCG_Status_t xjpcg_batch_solve_mixed(xjpcg_batch_t* outer, xjpcg_batch_t* inner, const double* d_diag,
                                                   const double* d_b, double* d_x, uint32_t max_outer, double tol,
                                                   double inner_tol, uint32_t inner_max_iter, int32_t* d_iters,
@@ -638,6 +666,12 @@ extern "C" XJPCG_Status_t xjpcg_batch_solve_mixed(xjpcg_batch_t* outer, xjpcg_ba
             double tin = 0.5;
             for (int q = 0; q < SR; ++q)
                 if (keep[q]) tin = std::min(tin, 0.5 * tol / res[q]);
+            // fp32 work vectors keep the true residual with the recursive one
+            // down to about 1e-7 (x stays fp64): a step asks them for no more
+            // than that, and the next step corrects the rest (on the
+            // 512-system batch a floor of 1e-5 took 2,912 batched iterations
+            // over two steps, 1e-7 took 2,496)
+            if (I.vector_bits() == 32) tin = std::max(tin, kW32StepTol);
             tin = std::min(std::max(tin, inner_tol), 0.5);
             xj::hip_check(hipMemcpyAsync(d_keep, keep.data(), SR * sizeof(int), hipMemcpyHostToDevice, s), "mask");
             O.mask(d_keep, r, s);  // finished pairs get a zero right-hand side: they do not run

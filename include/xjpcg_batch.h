@@ -66,10 +66,22 @@ XJPCG_Status_t xjpcg_batch_solve(xjpcg_batch_t* batch, const double* d_diag, con
                                  uint32_t max_iter, double tol, int32_t* d_iters, double* d_res,
                                  int64_t* h_launched, void* stream);
 
+/* Storage of the work vectors r, p and Ap of xjpcg_batch_solve: bits = 64
+ * (the default) or 32.  With 32 they are stored in fp32 -- every value is
+ * widened to fp64 before it is used; x, all arithmetic, the per-pair p.Ap,
+ * r.r and r.z sums, alpha and beta stay fp64 -- so an iteration moves about
+ * 40% fewer vector bytes, while the recursive residual keeps about seven
+ * significant digits: meant for the inner batch of xjpcg_batch_solve_mixed,
+ * whose fp64 outer loop (true residual, x += d) restores full accuracy.
+ * bits <= 0 only queries; *h_bits (may be NULL) receives the setting.  No
+ * counterpart in the reference. */
+XJPCG_Status_t xjpcg_batch_set_vectors(xjpcg_batch_t* batch, int32_t bits, int32_t* h_bits);
+
 /* Iterative refinement: `outer` (fp64-accurate values, same pattern and
  * sizes) computes the true residual r = b - A x in fp64; `inner` (typically
  * fp32 values) solves A d = r to max(inner_tol, 0.5 tol ||b||/||r||) for the
- * unfinished pair furthest from tol (capped at 0.5), finished pairs masked
+ * unfinished pair furthest from tol (capped at 0.5; at least 1e-7 when the
+ * inner batch has fp32 work vectors), finished pairs masked
  * to a zero right-hand side; x += d; repeat until every pair
  * has ||r||/||b|| < tol or max_outer steps.  d_res (fp64 [S][R], may be NULL)
  * receives the final true relative residuals. */

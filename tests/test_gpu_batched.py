@@ -161,3 +161,63 @@ def test_batched_diagonals_match_csr(cuda, split):
     assert B2.layout == 1
     with pytest.raises(XjpcgError):
         B2.set_layout(2)
+
+
+def test_batched_fp32_vectors_alone(cuda):
+    """fp32 work vectors (fp64 arithmetic and dots) on their own, to a tolerance fp32 storage can carry: every
+    pair stops within a few iterations of the oracle's fp64 JPCG and its true residual stays near tol."""
+    from hpc_amd import batched
+    mats, rhs = small_batch(count=4, nrhs=4)
+    R = 4
+    asm = batched.assemble(mats)
+    B = batched.Batch.from_host(asm, R, cuda)
+    assert B.set_vectors(32) == 32 and B.vector_bits == 32
+    tol, maxit = 1e-5, 5000
+    b = np.concatenate(rhs)
+    x, info = B.solve(_dev(cuda, asm["diag"]), _dev(cuda, b), maxit, tol)
+    xh = x.cpu().numpy()
+    its = info.iterations.cpu().numpy()
+    for s, A in enumerate(mats):
+        r0, r1 = asm["sys_row"][s], asm["sys_row"][s + 1]
+        for j in range(R):
+            ref = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, rhs[s][:, j].copy(), maxit, tol)
+            true = np.linalg.norm(rhs[s][:, j] - oracle.spmv(A.rowptr, A.col, A.val, xh[r0:r1, j].copy())) / \
+                np.linalg.norm(rhs[s][:, j])
+            _say(f"fp32 vectors: system {s} rhs {j}: {its[s, j]} it (oracle {ref['executed']}), true res {true:.2e}")
+            assert abs(int(its[s, j]) - ref["executed"]) <= max(3, ref["executed"] // 20)
+            assert true < 2 * tol
+
+
+def test_batched_mixed_fp32_vectors_matches_oracle(cuda):
+    """Refinement with an inner batch on fp32 values and fp32 work vectors: every pair reaches the true
+    relative residual tol in fp64, in at least two steps (each step is asked for at most 1e-7), and lands within
+    cond-bounded distance of the oracle's fp64 solution; the fp64-vector batch is unchanged by the switch."""
+    from hpc_amd import batched
+    mats, rhs = small_batch(count=5, nrhs=8)
+    R = 8
+    asm64 = batched.assemble(mats)
+    asm32 = batched.assemble(mats, split_fp32=True)
+    outer = batched.Batch.from_host(asm32, R, cuda)
+    inner = batched.Batch.from_host(asm32, R, cuda)
+    inner.set_vectors(32)
+    diag = _dev(cuda, asm64["diag"])
+    b = np.concatenate(rhs)
+    tol = 1e-8
+    xm, minfo, steps = batched.solve_mixed(outer, inner, diag, _dev(cuda, b), tol, inner_tol=0.0)
+    xmh = xm.cpu().numpy()
+    res = minfo.residual.cpu().numpy()
+    _say(f"mixed with fp32 vectors: {steps} steps, {minfo.launched} launched, worst true res {res.max():.2e}")
+    assert steps >= 2 and np.all(res < tol)
+    for s, A in enumerate(mats):
+        r0, r1 = asm64["sys_row"][s], asm64["sys_row"][s + 1]
+        for j in (0, R - 1):
+            ref = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, rhs[s][:, j].copy(), 5000, tol)
+            e = np.linalg.norm(xmh[r0:r1, j] - ref["x"]) / np.linalg.norm(ref["x"])
+            assert e <= 1e-5, (s, j, e)
+    # the same batch back on fp64 vectors gives the fp64 solve exactly
+    inner.set_vectors(64)
+    B64 = batched.Batch.from_host(asm32, R, cuda)
+    x1, i1 = inner.solve(diag, _dev(cuda, b), 5000, tol)
+    x2, i2 = B64.solve(diag, _dev(cuda, b), 5000, tol)
+    assert np.array_equal(x1.cpu().numpy(), x2.cpu().numpy())
+    assert np.array_equal(i1.iterations.cpu().numpy(), i2.iterations.cpu().numpy())

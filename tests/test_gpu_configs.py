@@ -125,28 +125,37 @@ def test_batch512x16_all_pairs_and_sampled_oracle(cuda):
     res_rec = info.residual.cpu().numpy()
     res_m = minfo.residual.cpu().numpy()
     xh, xmh = x.cpu().numpy(), xm.cpu().numpy()
-    del B64, outer, inner, x, xm, b
+    del inner, xm
+    # the inner batch on fp32 work vectors as well
+    inner32 = batched.Batch.from_host(asm32, R, cuda)
+    inner32.set_vectors(32)
+    t0 = time.time()
+    xw, winfo, wsteps = batched.solve_mixed(outer, inner32, diag, b, tol, inner_tol=0.0)
+    torch.cuda.synchronize()
+    tw = time.time() - t0
+    res_w = winfo.residual.cpu().numpy()
+    xwh = xw.cpu().numpy()
+    del B64, outer, inner32, x, xw, b
     _say(f"batch: fp64 {t64:.1f}s ({info.launched} launched, pair iterations {its.min()}..{its.max()}); mixed "
-         f"{tm:.1f}s ({steps} outer steps, {minfo.launched} launched)")
+         f"{tm:.1f}s ({steps} outer steps, {minfo.launched} launched); mixed with fp32 vectors {tw:.1f}s "
+         f"({wsteps} outer steps, {winfo.launched} launched, worst true res {res_w.max():.2e})")
     assert np.all(res_rec < tol)
     assert np.all(res_m < tol)
+    assert np.all(res_w < tol)
     # every pair's true residual ||b - A x|| / ||b||, with the oracle's SpMV on the whole block-diagonal CSR
     th = _threads()
-    worst64 = worstm = 0.0
+    worst = {"fp64": 0.0, "mixed": 0.0, "mixed_fp32_vectors": 0.0}
     for j in range(R):
         bj = np.ascontiguousarray(bh[:, j])
         bn = np.add.reduceat(bj * bj, sys_row[:-1])
-        for xs, tag in ((xh, "fp64"), (xmh, "mixed")):
+        for xs, tag in ((xh, "fp64"), (xmh, "mixed"), (xwh, "mixed_fp32_vectors")):
             r = bj - oracle.spmv(asm64["rowptr"], asm64["col"], asm64["val"], np.ascontiguousarray(xs[:, j]),
                                  threads=th)
             rel = np.sqrt(np.add.reduceat(r * r, sys_row[:-1]) / bn)
-            if tag == "fp64":
-                worst64 = max(worst64, float(rel.max()))
-            else:
-                worstm = max(worstm, float(rel.max()))
-    _say(f"batch: worst true residual over all {len(mats) * R} pairs: fp64 {worst64:.2e}, mixed {worstm:.2e}")
-    assert worst64 < 1.01 * tol
-    assert worstm < tol
+            worst[tag] = max(worst[tag], float(rel.max()))
+    _say(f"batch: worst true residual over all {len(mats) * R} pairs: {worst}")
+    assert worst["fp64"] < 1.01 * tol
+    assert worst["mixed"] < tol and worst["mixed_fp32_vectors"] < tol
     # oracle solves: the largest system (both ends of its right-hand sides), the smallest and the median
     sizes = np.diff(sys_row)
     order = np.argsort(sizes, kind="stable")
@@ -158,11 +167,13 @@ def test_batch512x16_all_pairs_and_sampled_oracle(cuda):
             ref = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, bh[r0:r1, j].copy(), maxit, tol, threads=th)
             e64 = _rel(xh[r0:r1, j], ref["x"])
             em = _rel(xmh[r0:r1, j], ref["x"])
+            ew = _rel(xwh[r0:r1, j], ref["x"])
             _say(f"system {s} (n={A.n}) rhs {j}: oracle {ref['executed']} it, gpu {its[s, j]}; fp64 rel err "
-                 f"{e64:.2e}, mixed {em:.2e}")
+                 f"{e64:.2e}, mixed {em:.2e}, mixed fp32 vectors {ew:.2e}")
             assert abs(int(its[s, j]) - ref["executed"]) <= 1
             assert e64 <= 1e-10
             assert em <= MIXED_ERR_BOUND_BATCH
+            assert ew <= MIXED_ERR_BOUND_BATCH
 
 
 def test_persistent_loop_fp32_operator_stencil160_vs_oracle(cuda):

@@ -340,6 +340,27 @@ def test_equal_matrices_give_identical_solves(cuda):
     assert np.linalg.norm(out[0][0] - o["x"]) / np.linalg.norm(o["x"]) <= 1e-10
 
 
+def test_equal_graph_matrices_give_identical_panel_solves(cuda):
+    """The random-graph workload at 300,000 rows (columns scattered, no int16 stream): every operator built on
+    it solves on the panel layout by rule, whatever its creation-time timings, so two operators give
+    bit-identical iterates and both match the oracle."""
+    from hpc_amd import device
+    A = workloads.graph_laplacian(300_000, seed=9)
+    b = workloads.uniform(9, 1, A.n, -1, 1)
+    out = []
+    for _ in range(2):
+        op = device.CsrOperator(_t(cuda, A.rowptr), _t(cuda, A.col), _t(cuda, A.val), rowptr_host=A.rowptr)
+        assert op.solve_layout == 2 and not op.compact
+        x, info = op.solve(_t(cuda, A.diag), _t(cuda, b), 20000, 1e-10)
+        out.append((x.cpu().numpy(), info.executed, info.residual))
+        op.close()
+    assert out[0][1] == out[1][1] and out[0][2] == out[1][2] and np.array_equal(out[0][0], out[1][0])
+    o = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, b, 20000, 1e-10, threads=8)
+    _say(f"graph n={A.n} on panels: {out[0][1]} iterations, oracle {o['executed']}")
+    assert abs(out[0][1] - o["executed"]) <= 1
+    assert np.linalg.norm(out[0][0] - o["x"]) / np.linalg.norm(o["x"]) <= 1e-10
+
+
 def test_diagonal_solve_7point_above_floor(cuda):
     """An anisotropic 3D 7-point operator with an odd number of rows above the diagonal-solve floor: the
     operator solves on the diagonals by itself (two rows per lane, the last lane's second row padding) and
