@@ -546,10 +546,11 @@ def batched_profile(args):
     b = torch.from_numpy(np.concatenate(rhs)).to(dev)
     diag = torch.from_numpy(asm["diag"]).to(dev)
     B = batched.Batch.from_host(asm, args.nrhs, dev)
+    B.set_vectors(args.batch_vectors)
     x, info = B.solve(diag, b, 9, 1e-30)
     torch.cuda.synchronize()
     print(json.dumps({"N": B.N, "nnz": int(asm["col"].size), "R": args.nrhs, "launched": info.launched,
-                      "config": B.info()}), flush=True)
+                      "vector_bits": B.vector_bits, "config": B.info()}), flush=True)
 
 
 def bench_ingest(args):
@@ -813,6 +814,8 @@ def main():
     ap.add_argument("--profile-layout", type=int, default=0)
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--nrhs", type=int, default=16)
+    ap.add_argument("--batch-vectors", type=int, default=64, choices=[32, 64],
+                    help="batched-profile: work vectors r, p, Ap in fp64 or fp32")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "one GPU only: the row-block split over several GPUs is designed, not "
