@@ -718,6 +718,23 @@ def bench_headline(args):
     err_vs64 = float(torch.linalg.norm(xm - x) / torch.linalg.norm(x))
     log(f"JPCG mixed: {mixed_ms:.1f} ms, outer {minfo.outer}, inner total {minfo.inner_total}, "
         f"true res {true_res_m:.2e}, |x_mixed - x_fp64|/|x_fp64| = {err_vs64:.2e}")
+    # ---- JPCG mixed-refined with fp32 work vectors in the inner operator (diagonal solve layout only)
+    mixed32 = None
+    if inner.solve_layout == 4:
+        inner.set_vectors(32)
+        wtimes, (xm, winfo) = _timed(lambda: device.solve_mixed(outer, inner, diag, b, sysm.tol,
+                                                                inner_tol=args.inner_tol, x=xm),
+                                     args.steps, args.warmup, stream)
+        inner.set_vectors(64)
+        w_ms = float(np.median(wtimes))
+        true_res_w = (op.residual(b, xm)[1] / bb) ** 0.5
+        err_w = float(torch.linalg.norm(xm - x) / torch.linalg.norm(x))
+        log(f"JPCG mixed, fp32 work vectors: {w_ms:.1f} ms, outer {winfo.outer}, inner total {winfo.inner_total}, "
+            f"true res {true_res_w:.2e}, |x - x_fp64|/|x_fp64| = {err_w:.2e}")
+        mixed32 = {"time_to_tol_ms": round(w_ms, 3), "outer": winfo.outer, "inner_total": winfo.inner_total,
+                   "inner_layout": LAYOUT_NAMES[inner.solve_layout], "inner_vectors": "r, p, Ap fp32; x, dots fp64",
+                   "true_res": true_res_w, "rel_diff_vs_fp64": err_w,
+                   "speedup_vs_fp64": round(fp64_ms / w_ms, 3)}
     xh = x.cpu().numpy()
 
     cpu = None
@@ -735,6 +752,7 @@ def bench_headline(args):
                "ms_1_thread": round(s_1 * 1e3, 1), "iterations": ref["executed"],
                "gpu_fp64_speedup": round(s_all * 1e3 / fp64_ms, 1),
                "gpu_mixed_speedup": round(s_all * 1e3 / mixed_ms, 1),
+               "gpu_mixed_fp32_vectors_speedup": round(s_all / mixed32["time_to_tol_ms"] * 1e3, 1) if mixed32 else None,
                "gpu_rel_err_vs_oracle": e}
         log(f"CPU: {cpu}")
 
@@ -771,6 +789,7 @@ def bench_headline(args):
                       "inner_layout": LAYOUT_NAMES[inner.solve_layout], "inner_tol": args.inner_tol,
                       "true_res": true_res_m, "rel_diff_vs_fp64": err_vs64},
             "speedup_mixed_vs_fp64": round(fp64_ms / mixed_ms, 3),
+            "mixed_fp32_vectors": mixed32,
         },
         "roofline": {"bound": "hbm", "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kname,

@@ -49,6 +49,7 @@ _SIGNATURES = {
     "xjpcg_op_set_path": (c_int, [c_void_p, c_int, POINTER(c_int)]),
     "xjpcg_op_set_layout": (c_int, [c_void_p, c_int, c_void_p, POINTER(c_int)]),
     "xjpcg_op_solve_layout": (c_int, [c_void_p, POINTER(c_int)]),
+    "xjpcg_op_set_vectors": (c_int, [c_void_p, c_int, POINTER(c_int)]),
     "xjpcg_panel_dims": (c_int, [c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
     "xjpcg_panel_plan": (c_int, [c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  POINTER(c_int)]),

@@ -154,6 +154,15 @@ __device__ __forceinline__ void load_q(const T* __restrict__ p, double (&out)[Q]
     for (int q = 0; q < Q; ++q) out[q] = (double)p[q];
 }
 
+// the same load kept in T; converting at the use lets loads issued under a
+// condition stay in flight together (a conversion under the condition waits
+// for its load)
+template <int Q, typename T>
+__device__ __forceinline__ void load_raw(const T* __restrict__ p, T (&out)[Q]) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) out[q] = p[q];
+}
+
 // acc[q] = row i of A times column (q0 + q) of the [N][R] block xin
 template <typename VT, int R, typename XT>
 __device__ __forceinline__ void brow(const BatchParams& P, const VT* __restrict__ val, const XT* __restrict__ xin,
@@ -217,24 +226,25 @@ __device__ __forceinline__ void brow_dia(const BatchParams& P, const BDiaSys& Y,
     // 2816 vs 3045 ms on 64 systems x 16 right-hand sides)
 #pragma unroll
     for (int h = 0; h < kBDiaMax; h += 4) {
-        double v[4];
-        double g[4][Q];
+        VT v[4];
+        XT g[4][Q];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int d = h + u;
             if (d < Y.nd) {
-                v[u] = (double)ld_stream(dval + Y.base + (size_t)d * ns + li);
+                v[u] = ld_stream(dval + Y.base + (size_t)d * ns + li);
                 int c = i + Y.off[d];
                 c = c < Y.r0 ? Y.r0 : (c >= Y.r1 ? Y.r1 - 1 : c);
-                load_q<Q>(xin + (size_t)c * R + q0, g[u]);
+                load_raw<Q>(xin + (size_t)c * R + q0, g[u]);
             }
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int d = h + u;
             if (d < Y.nd && ((m >> d) & 1u)) {
+                const double vu = (double)v[u];
 #pragma unroll
-                for (int q = 0; q < Q; ++q) acc[q] += v[u] * g[u][q];
+                for (int q = 0; q < Q; ++q) acc[q] += vu * (double)g[u][q];
             }
         }
     }

@@ -16,6 +16,9 @@ namespace {
 
 thread_local std::string g_last_error;
 
+// smallest relative tolerance a refinement step asks of an inner operator with fp32 work vectors
+constexpr double kW32StepTol = 1e-7;
+
 template <typename F>
 XJPCG_Status_t guarded(F f) {
     try {
@@ -143,6 +146,14 @@ XJPCG_Status_t xjpcg_op_solve_layout(const xjpcg_op_t* op, int* h_layout) {
     return guarded([&] {
         const xj::Operator& o = impl_of(const_cast<xjpcg_op_t*>(op));
         if (h_layout) *h_layout = o.solve_layout();
+    });
+}
+
+XJPCG_Status_t xjpcg_op_set_vectors(xjpcg_op_t* op, int bits, int* h_bits) {
+    return guarded([&] {
+        xj::Operator& o = impl_of(op);
+        if (bits != 0) o.set_vector_bits(bits);
+        if (h_bits) *h_bits = o.vector_bits();
     });
 }
 
@@ -275,6 +286,10 @@ XJPCG_Status_t xjpcg_solve_mixed(xjpcg_op_t* outer, xjpcg_op_t* inner, const dou
             // is only an approximation of A) and never looser than 0.5.
             double tin = std::max(inner_tol, 0.5 * tol / res);
             tin = std::min(tin, 0.5);
+            // fp32 work vectors: the recursive residual of the correction
+            // drifts from its true residual at the fp32 level, so a step is
+            // asked for no more than kW32StepTol
+            if (I.vector_bits() == 32) tin = std::max(tin, kW32StepTol);
             xj::SolveResult R = I.solve(d_diag, r, d, inner_max_iter, tin, nullptr, s);
             inner_total += R.executed;
             xj::hip_check(xj::launch_axpy(n, 1.0, d, d_x, s), "x += d");

@@ -171,21 +171,33 @@ __global__ __launch_bounds__(kBlock) void jpcg_persistent(JpcgParams P) {
 // kernels, so no kernel reads a field that is being written during its own
 // launch.  Once `stop` is set the remaining enqueued launches return at once.
 
+// Work vectors r, p, Ap are stored as PT: double, or float for the inner
+// operator of a refinement (P.w32).  Values are widened to double before any
+// arithmetic and rounded to PT only when stored; dot products use the stored
+// values and accumulate in double; x, J and the loop state stay double.
+// With PT = double every expression is the plain fp64 loop.
+template <typename PT>
+__device__ __forceinline__ PT* wv(double* p) { return reinterpret_cast<PT*>(p); }
+
+template <typename PT>
 __global__ __launch_bounds__(kBlock) void cg_setup(CgParams P) {
     __shared__ double red[kWaves];
     const int L = (int)blockIdx.x;
     const int stride = gridDim.x * kBlock;
+    PT* __restrict__ r = wv<PT>(P.r);
+    PT* __restrict__ p = wv<PT>(P.p);
     double dot = 0.0, rz = 0.0;
     for (int i = L * kBlock + threadIdx.x; i < P.n; i += stride) {
         const double bi = P.b[i];
+        const double ri = (double)(PT)bi;
         const double Ji = 1.0 / P.diag[i];
-        const double zi = Ji * bi;
-        P.r[i] = bi;
+        const double zi = Ji * ri;
+        r[i] = (PT)ri;
         P.J[i] = Ji;
-        P.p[i] = zi;
+        p[i] = (PT)zi;
         P.x[i] = 0.0;
-        dot += bi * bi;
-        rz += bi * zi;
+        dot += bi * bi;  // the stop rule's ||b|| from the exact right-hand side
+        rz += ri * zi;
     }
     dot = block_sum(dot, red);
     rz = block_sum(rz, red);
@@ -331,6 +343,7 @@ __device__ __forceinline__ double cg_alpha_of(const CgParams& P, uint32_t k, dou
     return P.st->rz[k & 1] / pap;
 }
 
+template <typename PT>
 __global__ __launch_bounds__(kBlock) void cg_update_r(CgParams P) {
     __shared__ double red[kWaves];
     CgState* st = P.st;
@@ -342,10 +355,12 @@ __global__ __launch_bounds__(kBlock) void cg_update_r(CgParams P) {
     }
     const double alpha = cg_alpha_of(P, st->k, red);
     const int stride = gridDim.x * kBlock;
+    PT* __restrict__ r = wv<PT>(P.r);
+    const PT* __restrict__ Ap = wv<PT>(P.Ap);
     double rr = 0.0, rz = 0.0;
     for (int i = L * kBlock + threadIdx.x; i < P.n; i += stride) {
-        const double ri = -alpha * P.Ap[i] + P.r[i];
-        P.r[i] = ri;
+        const double ri = (double)(PT)(-alpha * (double)Ap[i] + (double)r[i]);
+        r[i] = (PT)ri;
         const double zi = P.J[i] * ri;
         rr += ri * ri;
         rz += ri * zi;
@@ -359,6 +374,7 @@ __global__ __launch_bounds__(kBlock) void cg_update_r(CgParams P) {
     }
 }
 
+template <typename PT>
 __global__ __launch_bounds__(kBlock) void cg_update_xp(CgParams P) {
     __shared__ double red[kWaves];
     CgState* st = P.st;
@@ -371,10 +387,12 @@ __global__ __launch_bounds__(kBlock) void cg_update_xp(CgParams P) {
     const double rzn = grid_total(P.partC, P.G2, red);
     const double beta = rzn / rz;
     const int stride = gridDim.x * kBlock;
+    PT* __restrict__ p = wv<PT>(P.p);
+    const PT* __restrict__ r = wv<PT>(P.r);
     for (int i = L * kBlock + threadIdx.x; i < P.n; i += stride) {
-        const double pi = P.p[i];
+        const double pi = (double)p[i];
         P.x[i] = alpha * pi + P.x[i];
-        P.p[i] = beta * pi + P.J[i] * P.r[i];
+        p[i] = (PT)(beta * pi + P.J[i] * (double)r[i]);
     }
     if (L == 0 && threadIdx.x == 0) {
         st->rz[(k + 1) & 1] = rzn;
@@ -602,9 +620,9 @@ __device__ __forceinline__ void ld_rows(const T* p, T (&o)[R]) {
     }
 }
 
-template <typename VT>
+template <typename VT, typename XT = double>
 __device__ __forceinline__ void dia_rows_s(const DiaDev& D, int n, int i0, const double* __restrict__ dsplit,
-                                         const double* __restrict__ x, double (&s)[DiaLane<VT>::R]) {
+                                         const XT* __restrict__ x, double (&s)[DiaLane<VT>::R]) {
 #pragma clang fp contract(off)
     constexpr int R = DiaLane<VT>::R, C = DiaLane<VT>::C;
     const VT* __restrict__ val = static_cast<const VT*>(D.val);
@@ -616,12 +634,12 @@ __device__ __forceinline__ void dia_rows_s(const DiaDev& D, int n, int i0, const
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int i = i0 + r < n ? i0 + r : n - 1;
-        s[r] = dsplit ? dsplit[i] * x[i] : 0.0;
+        s[r] = dsplit ? dsplit[i] * (double)x[i] : 0.0;
     }
 #pragma nounroll
     for (int d0 = 0; d0 < nd; d0 += C) {
         VT v[C][R];
-        double g[C][R];
+        XT g[C][R];  // converted where it is used: a conversion here would wait for each load
 #pragma unroll
         for (int u = 0; u < C; ++u) {
             const int d = d0 + u;
@@ -641,7 +659,7 @@ __device__ __forceinline__ void dia_rows_s(const DiaDev& D, int n, int i0, const
             const int d = d0 + u;
 #pragma unroll
             for (int r = 0; r < R; ++r)
-                if (d < nd && ((m[r] >> d) & 1u)) s[r] += (double)v[u][r] * g[u][r];
+                if (d < nd && ((m[r] >> d) & 1u)) s[r] += (double)v[u][r] * (double)g[u][r];
         }
     }
 }
@@ -727,31 +745,36 @@ __global__ __launch_bounds__(kBlock) void spmv_dia(int n, DiaDev D, const double
 // lane (every address follows from the block index, so nothing waits on a
 // table); Ap is stored and each wavefront leaves one p.Ap partial, a lane's
 // rows' products added in row order, then an xor-butterfly within the
-// wavefront -- a fixed order, no barrier.
-template <typename VT>
+// wavefront -- a fixed order, no barrier.  With fp32 work vectors (PT =
+// float) p is widened as it is read, Ap is rounded as it is stored and the
+// partial uses the stored value.
+template <typename VT, typename PT>
 __global__ __launch_bounds__(kBlock) void cg_spmv_dia(CgParams P) {
     constexpr int R = DiaLane<VT>::R;
     CgState* st = P.st;
     const int stop = st->stop;
     const int b = (int)blockIdx.x;
     if (stop) return;
+    const PT* __restrict__ p = wv<PT>(P.p);
+    PT* __restrict__ Ap = wv<PT>(P.Ap);
     double pap = 0.0;
     const int i0 = (b * kBlock + (int)threadIdx.x) * R;
     if (i0 < P.n) {
         double pi[R], y[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) pi[r] = P.p[i0 + r < P.n ? i0 + r : P.n - 1];
-        if constexpr (R > 1) {
+        for (int r = 0; r < R; ++r) pi[r] = (double)p[i0 + r < P.n ? i0 + r : P.n - 1];
+        if constexpr (R > 1 && sizeof(PT) == 8) {
             if (P.n >= R) dia_rows_v<VT>(P.dia, P.n, i0, P.dsplit, P.p, y);
             else dia_rows_s<VT>(P.dia, P.n, i0, P.dsplit, P.p, y);
         } else {
-            dia_rows_s<VT>(P.dia, P.n, i0, P.dsplit, P.p, y);
+            dia_rows_s<VT, PT>(P.dia, P.n, i0, P.dsplit, p, y);
         }
 #pragma unroll
         for (int r = 0; r < R; ++r)
             if (i0 + r < P.n) {
-                P.Ap[i0 + r] = y[r];
-                pap += pi[r] * y[r];
+                const PT a = (PT)y[r];
+                Ap[i0 + r] = a;
+                pap += pi[r] * (double)a;
             }
     }
     pap = wave_sum(pap);
@@ -1035,7 +1058,11 @@ hipError_t launch_jpcg(const JpcgParams& P0, bool f32, bool c16, int G, hipStrea
 }
 
 hipError_t launch_cg_setup(const CgParams& P, hipStream_t s) {
-    cg_setup<<<P.G2, kBlock, 0, s>>>(P);
+    if (P.w32 && !P.dia_on) return hipErrorInvalidValue;
+    if (P.w32)
+        cg_setup<float><<<P.G2, kBlock, 0, s>>>(P);
+    else
+        cg_setup<double><<<P.G2, kBlock, 0, s>>>(P);
     cg_setup_finish<<<1, kBlock, 0, s>>>(P);
     return hipGetLastError();
 }
@@ -1049,10 +1076,17 @@ hipError_t launch_cg_iteration(const CgParams& P, bool f32, bool c16, hipStream_
             cg_spmv_panels<double><<<P.pan.nb, kPanelBlock, lds, s>>>(P);
     } else if (P.dia_on) {
         const int g = dia_groups(P.n, f32);
-        if (f32)
-            cg_spmv_dia<float><<<g, kBlock, 0, s>>>(P);
-        else
-            cg_spmv_dia<double><<<g, kBlock, 0, s>>>(P);
+        if (P.w32) {
+            if (f32)
+                cg_spmv_dia<float, float><<<g, kBlock, 0, s>>>(P);
+            else
+                cg_spmv_dia<double, float><<<g, kBlock, 0, s>>>(P);
+        } else {
+            if (f32)
+                cg_spmv_dia<float, double><<<g, kBlock, 0, s>>>(P);
+            else
+                cg_spmv_dia<double, double><<<g, kBlock, 0, s>>>(P);
+        }
     } else if (P.pat_on) {
         if (f32)
             cg_spmv_pattern<float><<<pattern_groups(P.tiles), kBlock, 0, s>>>(P);
@@ -1065,9 +1099,15 @@ hipError_t launch_cg_iteration(const CgParams& P, bool f32, bool c16, hipStream_
 }
 
 hipError_t launch_cg_vector_steps(const CgParams& P, hipStream_t s) {
+    if (P.w32 && !P.dia_on) return hipErrorInvalidValue;
     if (P.split_alpha) cg_alpha<<<P.nA2, kBlock, 0, s>>>(P);
-    cg_update_r<<<P.G2, kBlock, 0, s>>>(P);
-    cg_update_xp<<<P.G2, kBlock, 0, s>>>(P);
+    if (P.w32) {
+        cg_update_r<float><<<P.G2, kBlock, 0, s>>>(P);
+        cg_update_xp<float><<<P.G2, kBlock, 0, s>>>(P);
+    } else {
+        cg_update_r<double><<<P.G2, kBlock, 0, s>>>(P);
+        cg_update_xp<double><<<P.G2, kBlock, 0, s>>>(P);
+    }
     return hipGetLastError();
 }
 

@@ -137,6 +137,7 @@ struct CgParams {
     const double* b = nullptr;
     const double* diag = nullptr;
     double *x = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr, *J = nullptr;
+    int w32 = 0;                         // 1: r, p, Ap hold fp32 (diagonal layout only); x, J, dots fp64
     double *partA = nullptr, *partB = nullptr, *partC = nullptr;  // ntiles, G2, G2 doubles
     CgState* st = nullptr;
     Record* records = nullptr;

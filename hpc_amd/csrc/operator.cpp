@@ -639,7 +639,8 @@ SolveResult Operator::solve(const double* diag, const double* b, double* x, uint
     refresh_copies(s);
     int p = path_;
     if (p == kAuto) p = kMultiKernel;
-    if (!coop_ok_ || solve_layout_ == kPanels) p = kMultiKernel;  // the persistent kernel runs the tile SpMV only
+    // the persistent kernel runs the tile SpMV with fp64 vectors only
+    if (!coop_ok_ || solve_layout_ == kPanels || vector_bits() == 32) p = kMultiKernel;
     last_path_ = p;
     if (p == kPersistent) return solve_persistent(diag, b, x, max_iter, tol, records, s);
     return solve_multi(diag, b, x, max_iter, tol, records, s);
@@ -690,6 +691,7 @@ CgParams Operator::cg_params(const double* diag, const double* b, double* x, uin
     P.p = p_.ptr;
     P.Ap = Ap_.ptr;
     P.J = J_.ptr;
+    P.w32 = vector_bits() == 32 ? 1 : 0;
     P.partA = partA_.ptr;
     P.partB = partB_.ptr;
     P.partC = partC_.ptr;
@@ -698,6 +700,14 @@ CgParams Operator::cg_params(const double* diag, const double* b, double* x, uin
     P.tol = tol;
     P.max_rounds = max_iter + 1u;  // pcgImp.hpp:158 programs maxIter+1 rounds
     return P;
+}
+
+void Operator::set_vector_bits(int bits) {
+    if (bits != 32 && bits != 64) throw invalid("vector bits must be 32 or 64.");
+    if (bits == 32 && solve_layout_ != kDiagonals)
+        throw invalid("fp32 work vectors need the diagonal solve layout.");
+    w32_ = bits == 32;
+    drop_graph();
 }
 
 void Operator::drop_graph() {
@@ -739,7 +749,7 @@ SolveResult Operator::solve_multi(const double* diag, const double* b, double* x
     // graph captured for (x, records, max_rounds, values) replays for any b.
     if (graph_ok_ && (!gexec_ || gkey_.x != P.x || gkey_.records != P.records || gkey_.max_rounds != P.max_rounds ||
                       gkey_.val != P.val || gkey_.col != P.col || gkey_.pan_on != P.pan_on ||
-                      gkey_.pat_on != P.pat_on || gkey_.dia_on != P.dia_on))
+                      gkey_.pat_on != P.pat_on || gkey_.dia_on != P.dia_on || gkey_.w32 != P.w32))
         build_graph(P);
     // Enqueue iterations in chunks; after each chunk the stop flag is copied to
     // pinned memory.  The host checks chunk c-1's flag while chunk c is queued,

@@ -169,6 +169,13 @@ class Operator {
     int path() const { return path_; }
     int last_path() const { return last_path_; }
 
+    // Work vectors of the solves: 64 (fp64) or 32 (r, p and Ap stored in fp32;
+    // x, J, the arithmetic and the dot products stay fp64).  32 needs the
+    // diagonal solve layout and runs the multi-kernel loop; while another
+    // layout is in use the solves run with fp64 vectors.
+    void set_vector_bits(int bits);
+    int vector_bits() const { return w32_ && solve_layout_ == kDiagonals ? 32 : 64; }
+
   private:
     SolveResult solve_persistent(const double* diag, const double* b, double* x, uint32_t max_iter, double tol,
                                  void* records, hipStream_t s);
@@ -206,6 +213,7 @@ class Operator {
     int path_ = kAuto, last_path_ = 0;
     bool coop_ok_ = true;   // false once the runtime refused the cooperative launch
     bool graph_ok_ = true;  // false once graph capture failed; iterations are then launched directly
+    bool w32_ = false;      // fp32 work vectors requested
     int G_ = 0, G2_ = 0;
     DevBuf<double> r_, p_, Ap_, J_, part_, result_;
     DevBuf<double> partA_, partA2_, partB_, partC_;

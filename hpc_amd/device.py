@@ -139,6 +139,17 @@ class CsrOperator:
         check_dev(lib().xjpcg_op_solve_layout(self._h, ctypes.byref(act)))
         return act.value
 
+    def set_vectors(self, bits: int):
+        """Work vectors of the solves: 64, or 32 to store r, p and A p in fp32 (x, the preconditioner, the
+        arithmetic and the dot products stay fp64).  32 needs the diagonal solve layout."""
+        check_dev(lib().xjpcg_op_set_vectors(self._h, int(bits), None))
+
+    @property
+    def vector_bits(self) -> int:
+        bits = ctypes.c_int()
+        check_dev(lib().xjpcg_op_set_vectors(self._h, 0, ctypes.byref(bits)))
+        return bits.value
+
     PATH_AUTO, PATH_PERSISTENT, PATH_MULTI = 0, 1, 2
 
     def set_path(self, path: int):

@@ -88,6 +88,16 @@ XJPCG_Status_t xjpcg_op_set_layout(xjpcg_op_t* op, int layout, void* stream, int
  * from the standalone SpMV's layout.  xjpcg_op_set_layout with layout > 0
  * sets both; layout = 0 restores both creation-time choices. */
 XJPCG_Status_t xjpcg_op_solve_layout(const xjpcg_op_t* op, int* h_layout);
+/* Work vectors of the operator's solves: bits = 64 (default) or 32, which
+ * stores r, p and A p in fp32 while x, the preconditioner, the arithmetic and
+ * the dot products stay fp64 (meant for the inner operator of
+ * xjpcg_solve_mixed; halves the vector traffic of an iteration).  32 needs the
+ * diagonal solve layout (INVALID_VALUE otherwise) and runs the multi-kernel
+ * loop; while another solve layout is set later, the solves use fp64 vectors.
+ * bits = 0 queries.  *h_bits (may be NULL) receives the bits in effect.
+ * Extension of the device API: the reference's solver keeps all vectors in
+ * fp64 (pcg/sw/include/pcgImp.hpp:86-120). */
+XJPCG_Status_t xjpcg_op_set_vectors(xjpcg_op_t* op, int bits, int* h_bits);
 /* Host-only view of the panel layout an operator would build (no device
  * calls): xjpcg_panel_dims gives the rows per block, the row blocks and the
  * column panels for n rows; xjpcg_panel_plan fills seg[nb*P+1], off[nnz],

@@ -1,7 +1,8 @@
 """Oracle parity at the full sizes of the measured workloads.
 
 * 27-point stencil 160^3 (n = 4,096,000, 109M stored entries), tol 1e-10: the fp64 solve on the layout the
-  operator picks for its solves, and the mixed-refined solve, against the oracle's JPCG on the host cores.
+  operator picks for its solves, and the mixed-refined solve with fp64 and with fp32 work vectors in the
+  inner operator, against the oracle's JPCG on the host cores.
 * Graph Laplacian n = 4,000,000 (84.4M stored entries, lognormal degrees capped at 2000), tol 1e-10: the same.
 * 512 anisotropic diffusion systems x 16 right-hand sides (N = 44.1M rows, 266M entries), tol 1e-8: every
   pair's true residual from the oracle's SpMV, and oracle solves of sampled pairs that include the largest
@@ -50,7 +51,7 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def _fp64_and_mixed(cuda, sysm, label):
+def _fp64_and_mixed(cuda, sysm, label, fp32_vectors=False):
     import torch
     from hpc_amd import device
     A = sysm.A
@@ -71,6 +72,12 @@ def _fp64_and_mixed(cuda, sysm, label):
     true64 = (op.residual(b, x)[1] / bb) ** 0.5
     truem = (op.residual(b, xm)[1] / bb) ** 0.5
     xh, xmh = x.cpu().numpy(), xm.cpu().numpy()
+    if fp32_vectors:
+        inner.set_vectors(32)
+        xw, winfo = device.solve_mixed(outer, inner, diag, b, sysm.tol)
+        truew = (op.residual(b, xw)[1] / bb) ** 0.5
+        xwh = xw.cpu().numpy()
+        del xw
     del op, inner, outer
     th = _threads()
     t0 = time.time()
@@ -84,10 +91,16 @@ def _fp64_and_mixed(cuda, sysm, label):
     assert info.residual < sysm.tol and true64 < 1.01 * sysm.tol
     assert minfo.residual < sysm.tol and truem < sysm.tol
     assert em <= MIXED_ERR_BOUND
+    if fp32_vectors:
+        ew = _rel(xwh, ref["x"])
+        _say(f"{label}: mixed with fp32 work vectors outer {winfo.outer} inner {winfo.inner_total} "
+             f"true res {truew:.2e} rel err vs oracle {ew:.2e}")
+        assert winfo.residual < sysm.tol and truew < sysm.tol
+        assert ew <= MIXED_ERR_BOUND
 
 
 def test_stencil27_160_fp64_and_mixed_vs_oracle(cuda):
-    _fp64_and_mixed(cuda, workloads.c2_stencil27(160), "27-pt 160^3")
+    _fp64_and_mixed(cuda, workloads.c2_stencil27(160), "27-pt 160^3", fp32_vectors=True)
 
 
 def test_graph4m_fp64_and_mixed_vs_oracle(cuda):

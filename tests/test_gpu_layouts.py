@@ -401,6 +401,53 @@ def test_mixed_inner_on_diagonals_above_floor(cuda):
     assert err < 1e-8
 
 
+def test_mixed_inner_fp32_vectors_on_diagonals(cuda):
+    """fp32 work vectors (r, p, Ap stored fp32; x and the dots fp64) in the inner operator of the refinement
+    on the odd 7-point operator: the refined solve reaches 1e-10 in true residual and agrees with the fp64
+    oracle's x; a plain solve with fp32 vectors to 1e-6 stays within 2% of the oracle's iterations and its
+    true residual within 1.5x of the tolerance.  The flag needs the diagonal solve layout and lapses, to
+    fp64 vectors, when another solve layout is set."""
+    from hpc_amd import device
+    from hpc_amd._lib import XjpcgError
+    A = workloads.stencil(workloads.STENCIL_3D7, 91, 81, 81, [0.5, 1.5, 2.5])
+    b = workloads.uniform(6, 1, A.n, -1, 1)
+    srp, scol, sv32, sdiag, exact = device.split_fp32(A.rowptr, A.col, A.val)
+    inner = device.CsrOperator(_t(cuda, srp), _t(cuda, scol), _t(cuda, sv32), dsplit=_t(cuda, sdiag), rowptr_host=srp)
+    outer = device.CsrOperator(_t(cuda, A.rowptr), _t(cuda, A.col), _t(cuda, A.val), rowptr_host=A.rowptr)
+    assert inner.solve_layout == 4 and inner.vector_bits == 64
+    inner.set_vectors(32)
+    assert inner.vector_bits == 32
+    diag, bt = _t(cuda, A.diag), _t(cuda, b)
+    xm, info = device.solve_mixed(outer, inner, diag, bt, 1e-10)
+    xh = xm.cpu().numpy()
+    S = A.to_scipy()
+    true = np.linalg.norm(b - S @ xh) / np.linalg.norm(b)
+    o = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, b, 20000, 1e-10)
+    err = np.linalg.norm(xh - o["x"]) / np.linalg.norm(o["x"])
+    _say(f"mixed, fp32 vectors: outer {info.outer} inner {info.inner_total} (oracle {o['executed']}) "
+         f"true res {true:.2e} vs oracle {err:.2e}")
+    assert info.outer >= 1 and true < 1e-10
+    assert err < 1e-8
+    # a plain solve with fp32 vectors on the fp64-valued operator
+    outer.set_layout(4)
+    outer.set_vectors(32)
+    x6, i6 = outer.solve(diag, bt, 20000, 1e-6)
+    o6 = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, b, 20000, 1e-6)
+    t6 = np.linalg.norm(b - S @ x6.cpu().numpy()) / np.linalg.norm(b)
+    _say(f"fp32 vectors to 1e-6: {i6.executed} iterations (oracle {o6['executed']}), true res {t6:.2e}")
+    assert abs(i6.executed - o6["executed"]) <= max(2, o6["executed"] // 50)
+    assert i6.residual < 1e-6 and t6 < 1.5e-6
+    outer.set_layout(1)
+    assert outer.vector_bits == 64
+    with pytest.raises(XjpcgError):
+        outer.set_vectors(32)
+    outer.set_layout(4)
+    assert outer.vector_bits == 32
+    outer.set_vectors(64)
+    x64, i64 = outer.solve(diag, bt, 20000, 1e-6)
+    assert i64.executed == o6["executed"] or abs(i64.executed - o6["executed"]) <= 1
+
+
 def test_solve_layout_chosen_separately(cuda):
     """The solves keep their own creation-time layout; set_layout(L > 0) sets both, 0 restores both."""
     from hpc_amd import device
