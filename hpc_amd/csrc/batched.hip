@@ -353,9 +353,10 @@ __global__ __launch_bounds__(kMaxBlock) void bupdate_r(BatchParams P) {
     double rr = 0.0, rz = 0.0;
     for (int i = r0 + g; i < r1; i += G) {
         const size_t e = (size_t)i * R + j;
+        const double Ji = P.J[i];  // loaded before the store to r, which could alias it
         const double ri = (double)(WT)(-alpha * (double)Ap[e] + (double)r[e]);
         r[e] = (WT)ri;
-        const double zi = P.J[i] * ri;
+        const double zi = Ji * ri;
         rr += ri * ri;
         rz += ri * zi;
     }
@@ -432,9 +433,15 @@ __global__ __launch_bounds__(kMaxBlock) void bupdate_xp(BatchParams P) {
     const WT* __restrict__ r = wvec<WT>(P.r);
     for (int i = r0 + g; i < r1; i += G) {
         const size_t e = (size_t)i * R + j;
+        // every operand is loaded before the first store (x could alias J
+        // or r for all the compiler knows, which would order their loads
+        // after it: two memory round trips per element instead of one)
         const double pi = (double)p[e];
-        P.x[e] = alpha * pi + P.x[e];
-        p[e] = (WT)(beta * pi + P.J[i] * (double)r[e]);
+        const double xi = P.x[e];
+        const double Ji = P.J[i];
+        const double ri = (double)r[e];
+        P.x[e] = alpha * pi + xi;
+        p[e] = (WT)(beta * pi + Ji * ri);
     }
 }
 

@@ -359,9 +359,10 @@ __global__ __launch_bounds__(kBlock) void cg_update_r(CgParams P) {
     const PT* __restrict__ Ap = wv<PT>(P.Ap);
     double rr = 0.0, rz = 0.0;
     for (int i = L * kBlock + threadIdx.x; i < P.n; i += stride) {
+        const double Ji = P.J[i];  // loaded before the store to r, which could alias it
         const double ri = (double)(PT)(-alpha * (double)Ap[i] + (double)r[i]);
         r[i] = (PT)ri;
-        const double zi = P.J[i] * ri;
+        const double zi = Ji * ri;
         rr += ri * ri;
         rz += ri * zi;
     }
@@ -390,9 +391,15 @@ __global__ __launch_bounds__(kBlock) void cg_update_xp(CgParams P) {
     PT* __restrict__ p = wv<PT>(P.p);
     const PT* __restrict__ r = wv<PT>(P.r);
     for (int i = L * kBlock + threadIdx.x; i < P.n; i += stride) {
+        // every operand is loaded before the first store (x could alias J or
+        // r for all the compiler knows, which would order their loads after
+        // it: two memory round trips per element instead of one)
         const double pi = (double)p[i];
-        P.x[i] = alpha * pi + P.x[i];
-        p[i] = (PT)(beta * pi + P.J[i] * (double)r[i]);
+        const double xi = P.x[i];
+        const double Ji = P.J[i];
+        const double ri = (double)r[i];
+        P.x[i] = alpha * pi + xi;
+        p[i] = (PT)(beta * pi + Ji * ri);
     }
     if (L == 0 && threadIdx.x == 0) {
         st->rz[(k + 1) & 1] = rzn;
@@ -607,6 +614,7 @@ struct DiaLane {
 static_assert(DiaLane<double>::R == dia_rows_per_lane(false) && DiaLane<float>::R == dia_rows_per_lane(true),
               "rows per lane agree with the host's grid");
 static_assert(kDiaBlock == kBlock, "diagonal kernels run kBlock lanes");
+static_assert(kVectorBlock == kBlock, "vector kernels run kBlock lanes");
 
 template <typename T, int R>
 __device__ __forceinline__ void ld_rows(const T* p, T (&o)[R]) {

@@ -48,6 +48,25 @@ constexpr int dia_groups(int n, bool f32) {
     return (n + kDiaBlock * dia_rows_per_lane(f32) - 1) / (kDiaBlock * dia_rows_per_lane(f32));
 }
 constexpr int dia_cg_partials(int n, bool f32) { return dia_groups(n, f32) * (kDiaBlock / 64); }
+// Workgroups of the CG vector kernels (256 lanes, grid-stride; one r.r / r.z
+// partial each, and every workgroup of the x/p update re-sums all of them):
+// one row per lane up to 256 workgroups, so a small system's update is one
+// memory round trip per lane (256^2 Laplacian: 13.1 us per iteration against
+// 14.3 with n/1024 = 64 workgroups); beyond that n/1024, up to 1024 (more
+// workgroups cost more redundant partial sums than they save: 512^2 diffusion
+// 64.0 ms with 1024 against 56.8 with 256; 1024 measured best on the 160^3
+// stencil, fp64 solve 74.6 ms; 2048: 75.4, 4096: 79.9, 512: 75.8).
+constexpr int kVectorBlock = 256;
+constexpr int kMaxVectorGroups = 1024;
+constexpr int vector_groups(int n) {
+    const int per_lane = (n + kVectorBlock - 1) / kVectorBlock;
+    const int small = per_lane < 256 ? per_lane : 256;
+    const int large = (n + 1023) / 1024;
+    const int g = small > large ? small : large;
+    return g < 1 ? 1 : (g > kMaxVectorGroups ? kMaxVectorGroups : g);
+}
+static_assert(vector_groups(65536) == 256 && vector_groups(262144) == 256 && vector_groups(4096000) == 1024,
+              "vector grid rule");
 struct DiaDev {
     int ndia = 0;
     int ld = 0;                      // dia_ld(n)

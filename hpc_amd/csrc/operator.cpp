@@ -198,11 +198,7 @@ Operator::Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t*
     A_.tile_row = tiles_.ptr;
     A_.tile_k = tilek_.ptr;
     A_.ntiles = (int)t.size() - 1;
-    // vector-kernel workgroups (grid-stride, one r.r / r.z partial each, every
-    // workgroup of the x/p update re-sums all of them): 1024 measured best on
-    // the 160^3 stencil (fp64 solve 74.6 ms; 2048: 75.4, 4096: 79.9, 512: 75.8)
-    G2_ = std::min((n + 1023) / 1024, 1024);
-    if (G2_ < 1) G2_ = 1;
+    G2_ = vector_groups(n);
     partA_.ensure(std::max(1, A_.ntiles));
     partA2_.ensure(kAlphaBlocks);
     partB_.ensure(G2_);

@@ -27,7 +27,7 @@ namespace xj {
 
 constexpr int kStaticMaxRows = XJPCG_STATIC_MAX_ROWS;
 constexpr int kStaticSpmvBlocks = (kStaticMaxRows + kBlock - 1) / kBlock;  // p.Ap partials
-constexpr int kStaticG2 = (kStaticMaxRows + 1023) / 1024;                  // vector-kernel workgroups
+constexpr int kStaticG2 = vector_groups(kStaticMaxRows);                   // vector-kernel workgroups
 // r, p, Ap, J; the SpMV partials; the r.r and r.z partials; CgState (in doubles)
 constexpr size_t kStaticDoubles = 4 * (size_t)kStaticMaxRows + kStaticSpmvBlocks + 2 * kStaticG2 +
                                   (sizeof(CgState) + 7) / 8;
@@ -94,8 +94,7 @@ static StaticSolve csr_solve_static(int32_t n, const int32_t* rowptr, const int3
     P.n = n;
     P.ntiles = (n + kBlock - 1) / kBlock;
     P.tiles = P.ntiles;
-    P.G2 = std::min((n + 1023) / 1024, kStaticG2);
-    if (P.G2 < 1) P.G2 = 1;
+    P.G2 = vector_groups(n);
     P.split_alpha = 0;
     P.rowptr = rowptr;
     P.col = col;
