@@ -732,7 +732,7 @@ def bench_headline(args):
         log(f"JPCG mixed, fp32 work vectors: {w_ms:.1f} ms, outer {winfo.outer}, inner total {winfo.inner_total}, "
             f"true res {true_res_w:.2e}, |x - x_fp64|/|x_fp64| = {err_w:.2e}")
         mixed32 = {"time_to_tol_ms": round(w_ms, 3), "outer": winfo.outer, "inner_total": winfo.inner_total,
-                   "inner_layout": LAYOUT_NAMES[inner.solve_layout], "inner_vectors": "r, p, Ap fp32; x, dots fp64",
+                   "inner_layout": LAYOUT_NAMES[inner.solve_layout], "inner_vectors": "r, p, Ap, J fp32; x, dots fp64",
                    "true_res": true_res_w, "rel_diff_vs_fp64": err_w,
                    "speedup_vs_fp64": round(fp64_ms / w_ms, 3)}
     xh = x.cpu().numpy()

@@ -172,10 +172,11 @@ __global__ __launch_bounds__(kBlock) void jpcg_persistent(JpcgParams P) {
 // launch.  Once `stop` is set the remaining enqueued launches return at once.
 
 // Work vectors r, p, Ap are stored as PT: double, or float for the inner
-// operator of a refinement (P.w32).  Values are widened to double before any
-// arithmetic and rounded to PT only when stored; dot products use the stored
-// values and accumulate in double; x, J and the loop state stay double.
-// With PT = double every expression is the plain fp64 loop.
+// operator of a refinement (P.w32), which then also keeps the Jacobi factors
+// J = 1/diag in PT.  Values are widened to double before any arithmetic and
+// rounded to PT only when stored; dot products use the stored values and
+// accumulate in double; x and the loop state stay double.  With PT = double
+// every expression is the plain fp64 loop.
 template <typename PT>
 __device__ __forceinline__ PT* wv(double* p) { return reinterpret_cast<PT*>(p); }
 
@@ -186,14 +187,15 @@ __global__ __launch_bounds__(kBlock) void cg_setup(CgParams P) {
     const int stride = gridDim.x * kBlock;
     PT* __restrict__ r = wv<PT>(P.r);
     PT* __restrict__ p = wv<PT>(P.p);
+    PT* __restrict__ J = wv<PT>(P.J);
     double dot = 0.0, rz = 0.0;
     for (int i = L * kBlock + threadIdx.x; i < P.n; i += stride) {
         const double bi = P.b[i];
         const double ri = (double)(PT)bi;
-        const double Ji = 1.0 / P.diag[i];
+        const double Ji = (double)(PT)(1.0 / P.diag[i]);
         const double zi = Ji * ri;
         r[i] = (PT)ri;
-        P.J[i] = Ji;
+        J[i] = (PT)Ji;
         p[i] = (PT)zi;
         P.x[i] = 0.0;
         dot += bi * bi;  // the stop rule's ||b|| from the exact right-hand side
@@ -359,7 +361,7 @@ __global__ __launch_bounds__(kBlock) void cg_update_r(CgParams P) {
     const PT* __restrict__ Ap = wv<PT>(P.Ap);
     double rr = 0.0, rz = 0.0;
     for (int i = L * kBlock + threadIdx.x; i < P.n; i += stride) {
-        const double Ji = P.J[i];  // loaded before the store to r, which could alias it
+        const double Ji = (double)wv<PT>(P.J)[i];  // loaded before the store to r, which could alias it
         const double ri = (double)(PT)(-alpha * (double)Ap[i] + (double)r[i]);
         r[i] = (PT)ri;
         const double zi = Ji * ri;
@@ -396,7 +398,7 @@ __global__ __launch_bounds__(kBlock) void cg_update_xp(CgParams P) {
         // it: two memory round trips per element instead of one)
         const double pi = (double)p[i];
         const double xi = P.x[i];
-        const double Ji = P.J[i];
+        const double Ji = (double)wv<PT>(P.J)[i];
         const double ri = (double)r[i];
         P.x[i] = alpha * pi + xi;
         p[i] = (PT)(beta * pi + Ji * ri);

@@ -169,8 +169,9 @@ class Operator {
     int path() const { return path_; }
     int last_path() const { return last_path_; }
 
-    // Work vectors of the solves: 64 (fp64) or 32 (r, p and Ap stored in fp32;
-    // x, J, the arithmetic and the dot products stay fp64).  32 needs the
+    // Work vectors of the solves: 64 (fp64) or 32 (r, p, Ap and the Jacobi
+    // factors stored in fp32; x, the arithmetic and the dot products stay
+    // fp64).  32 needs the
     // diagonal solve layout and runs the multi-kernel loop; while another
     // layout is in use the solves run with fp64 vectors.
     void set_vector_bits(int bits);

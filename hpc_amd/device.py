@@ -140,7 +140,7 @@ class CsrOperator:
         return act.value
 
     def set_vectors(self, bits: int):
-        """Work vectors of the solves: 64, or 32 to store r, p and A p in fp32 (x, the preconditioner, the
+        """Work vectors of the solves: 64, or 32 to store r, p, A p and the Jacobi factors in fp32 (x, the
         arithmetic and the dot products stay fp64).  32 needs the diagonal solve layout."""
         check_dev(lib().xjpcg_op_set_vectors(self._h, int(bits), None))
 

@@ -89,8 +89,8 @@ XJPCG_Status_t xjpcg_op_set_layout(xjpcg_op_t* op, int layout, void* stream, int
  * sets both; layout = 0 restores both creation-time choices. */
 XJPCG_Status_t xjpcg_op_solve_layout(const xjpcg_op_t* op, int* h_layout);
 /* Work vectors of the operator's solves: bits = 64 (default) or 32, which
- * stores r, p and A p in fp32 while x, the preconditioner, the arithmetic and
- * the dot products stay fp64 (meant for the inner operator of
+ * stores r, p, A p and the Jacobi factors 1/diag in fp32 while x, the
+ * arithmetic and the dot products stay fp64 (meant for the inner operator of
  * xjpcg_solve_mixed; halves the vector traffic of an iteration).  32 needs the
  * diagonal solve layout (INVALID_VALUE otherwise) and runs the multi-kernel
  * loop; while another solve layout is set later, the solves use fp64 vectors.

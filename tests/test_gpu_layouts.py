@@ -402,7 +402,7 @@ def test_mixed_inner_on_diagonals_above_floor(cuda):
 
 
 def test_mixed_inner_fp32_vectors_on_diagonals(cuda):
-    """fp32 work vectors (r, p, Ap stored fp32; x and the dots fp64) in the inner operator of the refinement
+    """fp32 work vectors (r, p, Ap and J stored fp32; x and the dots fp64) in the inner operator of the refinement
     on the odd 7-point operator: the refined solve reaches 1e-10 in true residual and agrees with the fp64
     oracle's x; a plain solve with fp32 vectors to 1e-6 stays within 2% of the oracle's iterations and its
     true residual within 1.5x of the tolerance.  The flag needs the diagonal solve layout and lapses, to
