@@ -340,14 +340,18 @@ __global__ __launch_bounds__(kBlock) void cg_alpha(CgParams P) {
     if (threadIdx.x == 0) P.partA2[b] = s;
 }
 
-__device__ __forceinline__ double cg_alpha_of(const CgParams& P, uint32_t k, double* red) {
-    const double pap = P.split_alpha ? grid_total(P.partA2, P.nA2, red) : grid_total(P.partA, P.ntiles, red);
-    return P.st->rz[k & 1] / pap;
-}
+// The p.Ap partials: the SpMV's, or cg_alpha's fold of them.
+__device__ __forceinline__ const double* pap_parts(const CgParams& P) { return P.split_alpha ? P.partA2 : P.partA; }
+__device__ __forceinline__ int pap_count(const CgParams& P) { return P.split_alpha ? P.nA2 : P.ntiles; }
 
+// The vector kernels load their first row's operands before they sum the
+// partials they need (alpha; beta and the stop rule), so those memory round
+// trips overlap instead of following one another; the partial sets are then
+// reduced together with one pair of barriers.  Sums and their order are
+// those of one grid_total per set.
 template <typename PT>
 __global__ __launch_bounds__(kBlock) void cg_update_r(CgParams P) {
-    __shared__ double red[kWaves];
+    __shared__ double red[2 * kWaves];
     CgState* st = P.st;
     const int L = (int)blockIdx.x;
     const int stop = st->stop;
@@ -355,53 +359,87 @@ __global__ __launch_bounds__(kBlock) void cg_update_r(CgParams P) {
         if (L == 0 && threadIdx.x == 0) st->stop_k2 = stop;
         return;
     }
-    const double alpha = cg_alpha_of(P, st->k, red);
     const int stride = gridDim.x * kBlock;
+    const int i0 = L * kBlock + threadIdx.x;
     PT* __restrict__ r = wv<PT>(P.r);
     const PT* __restrict__ Ap = wv<PT>(P.Ap);
-    double rr = 0.0, rz = 0.0;
-    for (int i = L * kBlock + threadIdx.x; i < P.n; i += stride) {
-        const double Ji = (double)wv<PT>(P.J)[i];  // loaded before the store to r, which could alias it
-        const double ri = (double)(PT)(-alpha * (double)Ap[i] + (double)r[i]);
-        r[i] = (PT)ri;
-        const double zi = Ji * ri;
-        rr += ri * ri;
-        rz += ri * zi;
+    const PT* __restrict__ J = wv<PT>(P.J);
+    double a0 = 0.0, r0 = 0.0, J0 = 0.0;
+    if (i0 < P.n) {
+        a0 = (double)Ap[i0];
+        r0 = (double)r[i0];
+        J0 = (double)J[i0];
     }
-    rr = block_sum(rr, red);
-    rz = block_sum(rz, red);
+    double pap[1] = {strided_part(pap_parts(P), pap_count(P))};
+    block_sums<1>(pap, red);
+    const double alpha = st->rz[st->k & 1] / pap[0];
+    double acc[2] = {0.0, 0.0};  // r.r, r.z
+    // every operand of a row is loaded before its store to r (J could alias r
+    // for all the compiler knows, which would order its load after the store)
+    // (contractions written out: the compiler may otherwise fuse a different
+    // product in the peeled first row than in the loop)
+    auto row = [&](int i, double a, double rv, double Jv) {
+        const double ri = (double)(PT)__builtin_fma(-alpha, a, rv);
+        r[i] = (PT)ri;
+        const double zi = Jv * ri;
+        acc[0] = __builtin_fma(ri, ri, acc[0]);
+        acc[1] = __builtin_fma(ri, zi, acc[1]);
+    };
+    if (i0 < P.n) row(i0, a0, r0, J0);
+    for (int i = i0 + stride; i < P.n; i += stride) {
+        const double a = (double)Ap[i], rv = (double)r[i], Jv = (double)J[i];
+        row(i, a, rv, Jv);
+    }
+    block_sums<2>(acc, red);
     if (threadIdx.x == 0) {
-        P.partB[L] = rr;
-        P.partC[L] = rz;
+        P.partB[L] = acc[0];
+        P.partC[L] = acc[1];
         if (L == 0) st->stop_k2 = 0;  // stored last, after every load of this workgroup
     }
 }
 
 template <typename PT>
 __global__ __launch_bounds__(kBlock) void cg_update_xp(CgParams P) {
-    __shared__ double red[kWaves];
+    __shared__ double red[3 * kWaves];
     CgState* st = P.st;
     if (st->stop_k2) return;
     const int L = (int)blockIdx.x;
     const uint32_t k = st->k;
     const double rz = st->rz[k & 1];
-    const double alpha = cg_alpha_of(P, k, red);
-    const double rr = grid_total(P.partB, P.G2, red);
-    const double rzn = grid_total(P.partC, P.G2, red);
-    const double beta = rzn / rz;
     const int stride = gridDim.x * kBlock;
+    const int i0 = L * kBlock + threadIdx.x;
     PT* __restrict__ p = wv<PT>(P.p);
     const PT* __restrict__ r = wv<PT>(P.r);
-    for (int i = L * kBlock + threadIdx.x; i < P.n; i += stride) {
-        // every operand is loaded before the first store (x could alias J or
-        // r for all the compiler knows, which would order their loads after
-        // it: two memory round trips per element instead of one)
-        const double pi = (double)p[i];
-        const double xi = P.x[i];
-        const double Ji = (double)wv<PT>(P.J)[i];
-        const double ri = (double)r[i];
-        P.x[i] = alpha * pi + xi;
-        p[i] = (PT)(beta * pi + Ji * ri);
+    const PT* __restrict__ J = wv<PT>(P.J);
+    double p0 = 0.0, x0 = 0.0, J0 = 0.0, r0 = 0.0;
+    if (i0 < P.n) {
+        p0 = (double)p[i0];
+        x0 = P.x[i0];
+        J0 = (double)J[i0];
+        r0 = (double)r[i0];
+    }
+    const double* pa = pap_parts(P);
+    const int na = pap_count(P);
+    double sums[3] = {strided_head(pa, na), strided_head(P.partB, P.G2), strided_head(P.partC, P.G2)};
+    sums[0] = strided_tail(pa, na, sums[0]);
+    sums[1] = strided_tail(P.partB, P.G2, sums[1]);
+    sums[2] = strided_tail(P.partC, P.G2, sums[2]);
+    block_sums<3>(sums, red);
+    const double alpha = rz / sums[0];
+    const double rr = sums[1];
+    const double rzn = sums[2];
+    const double beta = rzn / rz;
+    // every operand of a row is loaded before its first store (x could alias
+    // J or r for all the compiler knows, which would order their loads after
+    // it: two memory round trips per row instead of one)
+    auto row = [&](int i, double pi, double xi, double Ji, double ri) {
+        P.x[i] = __builtin_fma(alpha, pi, xi);
+        p[i] = (PT)__builtin_fma(beta, pi, Ji * ri);
+    };
+    if (i0 < P.n) row(i0, p0, x0, J0, r0);
+    for (int i = i0 + stride; i < P.n; i += stride) {
+        const double pi = (double)p[i], xi = P.x[i], Ji = (double)J[i], ri = (double)r[i];
+        row(i, pi, xi, Ji, ri);
     }
     if (L == 0 && threadIdx.x == 0) {
         st->rz[(k + 1) & 1] = rzn;

@@ -64,11 +64,59 @@ __device__ __forceinline__ double block_sum_w(double v, double* red) {
 }
 __device__ __forceinline__ double block_sum(double v, double* red) { return block_sum_w<kWaves>(v, red); }
 
+// K sums over a workgroup with one pair of __syncthreads: value j is summed
+// exactly as block_sum sums it alone (wavefront butterfly, then wavefronts
+// 0..kWaves-1 in order).  red must hold K * kWaves doubles of LDS.
+template <int K>
+__device__ __forceinline__ void block_sums(double (&v)[K], double* red) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        v[j] = wave_sum(v[j]);
+        if (lane == 0) red[j * kWaves + w] = v[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        double s = red[j * kWaves];
+#pragma unroll
+        for (int i = 1; i < kWaves; ++i) s += red[j * kWaves + i];
+        v[j] = s;
+    }
+    __syncthreads();
+}
+
+// This thread's share of part[0..G), G >= 1: 0.0 + part[threadIdx.x] +
+// part[threadIdx.x + kBlock] + ... in that order, split in two so that
+// several sets' first loads can be in flight together.  The head load uses a
+// clamped index and no branch (a load inside a branch is waited for before
+// the branch ends); the tail issues four loads before adding them in order.
+__device__ __forceinline__ double strided_head(const double* part, int G) {
+    const int t = (int)threadIdx.x;
+    const double v = part[t < G ? t : G - 1];
+    return 0.0 + (t < G ? v : 0.0);
+}
+__device__ __forceinline__ double strided_tail(const double* part, int G, double s) {
+    for (int i = (int)threadIdx.x + kBlock; i < G; i += 4 * kBlock) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = i + u * kBlock;
+            v[u] = part[j < G ? j : G - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i + u * kBlock < G) s += v[u];
+    }
+    return s;
+}
+__device__ __forceinline__ double strided_part(const double* part, int G) {
+    return strided_tail(part, G, strided_head(part, G));
+}
+
 // Sum of part[0..G) in a fixed order, identical in every workgroup.
 __device__ __forceinline__ double grid_total(const double* part, int G, double* red) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < G; i += kBlock) s += part[i];
-    return block_sum(s, red);
+    return block_sum(strided_part(part, G), red);
 }
 
 // Grid-wide barrier for cooperatively launched kernels (all workgroups
