@@ -22,6 +22,7 @@
 namespace xj {
 
 constexpr int kMaxBlock = 512;
+constexpr int kRowsAhead = 4;  // rows a vector-kernel thread loads before it stores
 
 // Sum over the row groups of a workgroup for each lane j < R; lanes with
 // g == 0 return the total (fixed order g = 0, 1, ...).
@@ -351,14 +352,31 @@ __global__ __launch_bounds__(kMaxBlock) void bupdate_r(BatchParams P) {
     WT* __restrict__ r = wvec<WT>(P.r);
     const WT* __restrict__ Ap = wvec<WT>(P.Ap);
     double rr = 0.0, rz = 0.0;
-    for (int i = r0 + g; i < r1; i += G) {
-        const size_t e = (size_t)i * R + j;
-        const double Ji = P.J[i];  // loaded before the store to r, which could alias it
-        const double ri = (double)(WT)(-alpha * (double)Ap[e] + (double)r[e]);
-        r[e] = (WT)ri;
-        const double zi = Ji * ri;
-        rr += ri * ri;
-        rz += ri * zi;
+    // four of this thread's rows at a time: their operands are all loaded
+    // (clamped into the tile, no branch) before the first store to r, which
+    // the compiler must otherwise keep ahead of the next row's loads; rows
+    // are then updated and summed in order (contractions written out as fma)
+    for (int i = r0 + g; i < r1; i += kRowsAhead * G) {
+        double Jv[kRowsAhead], av[kRowsAhead], rv[kRowsAhead];
+#pragma unroll
+        for (int u = 0; u < kRowsAhead; ++u) {
+            const int iu = i + u * G < r1 ? i + u * G : i;
+            const size_t e = (size_t)iu * R + j;
+            Jv[u] = P.J[iu];
+            av[u] = (double)Ap[e];
+            rv[u] = (double)r[e];
+        }
+#pragma unroll
+        for (int u = 0; u < kRowsAhead; ++u) {
+            if (i + u * G < r1) {
+                const size_t e = (size_t)(i + u * G) * R + j;
+                const double ri = (double)(WT)__builtin_fma(-alpha, av[u], rv[u]);
+                r[e] = (WT)ri;
+                const double zi = Jv[u] * ri;
+                rr = __builtin_fma(ri, ri, rr);
+                rz = __builtin_fma(ri, zi, rz);
+            }
+        }
     }
     rr = group_sum<R>(rr, red);
     rz = group_sum<R>(rz, red);
@@ -431,17 +449,28 @@ __global__ __launch_bounds__(kMaxBlock) void bupdate_xp(BatchParams P) {
     const double alpha = P.alpha[q], beta = P.beta[q];
     WT* __restrict__ p = wvec<WT>(P.p);
     const WT* __restrict__ r = wvec<WT>(P.r);
-    for (int i = r0 + g; i < r1; i += G) {
-        const size_t e = (size_t)i * R + j;
-        // every operand is loaded before the first store (x could alias J
-        // or r for all the compiler knows, which would order their loads
-        // after it: two memory round trips per element instead of one)
-        const double pi = (double)p[e];
-        const double xi = P.x[e];
-        const double Ji = P.J[i];
-        const double ri = (double)r[e];
-        P.x[e] = alpha * pi + xi;
-        p[e] = (WT)(beta * pi + Ji * ri);
+    // four of this thread's rows at a time, every operand loaded (clamped into
+    // the tile, no branch) before the first store: x could alias J or r for
+    // all the compiler knows, which would order the next loads after it
+    for (int i = r0 + g; i < r1; i += kRowsAhead * G) {
+        double pv[kRowsAhead], xv[kRowsAhead], Jv[kRowsAhead], rv[kRowsAhead];
+#pragma unroll
+        for (int u = 0; u < kRowsAhead; ++u) {
+            const int iu = i + u * G < r1 ? i + u * G : i;
+            const size_t e = (size_t)iu * R + j;
+            pv[u] = (double)p[e];
+            xv[u] = P.x[e];
+            Jv[u] = P.J[iu];
+            rv[u] = (double)r[e];
+        }
+#pragma unroll
+        for (int u = 0; u < kRowsAhead; ++u) {
+            if (i + u * G < r1) {
+                const size_t e = (size_t)(i + u * G) * R + j;
+                P.x[e] = __builtin_fma(alpha, pv[u], xv[u]);
+                p[e] = (WT)__builtin_fma(beta, pv[u], Jv[u] * rv[u]);
+            }
+        }
     }
 }
 
