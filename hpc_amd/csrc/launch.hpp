@@ -51,8 +51,9 @@ constexpr int dia_cg_partials(int n, bool f32) { return dia_groups(n, f32) * (kD
 // Workgroups of the CG vector kernels (256 lanes, grid-stride; one r.r / r.z
 // partial each, and every workgroup of the x/p update re-sums all of them):
 // one row per lane up to 256 workgroups, so a small system's update is one
-// memory round trip per lane (256^2 Laplacian: 13.1 us per iteration against
-// 14.3 with n/1024 = 64 workgroups); beyond that n/1024, up to 1024 (more
+// memory round trip per lane (256^2 Laplacian, with the load-order change of
+// the vector kernels: 13.0 us per iteration against 14.6 with n/1024 = 64
+// workgroups); beyond that n/1024, up to 1024 (more
 // workgroups cost more redundant partial sums than they save: 512^2 diffusion
 // 64.0 ms with 1024 against 56.8 with 256; 1024 measured best on the 160^3
 // stencil, fp64 solve 74.6 ms; 2048: 75.4, 4096: 79.9, 512: 75.8).
