@@ -295,9 +295,20 @@ def bench_graph(args):
     bb = float(torch.dot(b, b))
     true64 = (op.residual(b, x)[1] / bb) ** 0.5
     truem = (op.residual(b, xm)[1] / bb) ** 0.5
+    xmh = xm.cpu().numpy()
+    # fp32 work vectors in the inner operator (panel layout): p gathered in fp32
+    mixed32 = None
+    if inner.solve_layout in (2, 4):
+        inner.set_vectors(32)
+        msw, (xm, winfo) = _timed(lambda: device.solve_mixed(outer, inner, diag, b, sysm.tol,
+                                                             inner_tol=args.inner_tol, x=xm), steps, warm, stream)
+        inner.set_vectors(64)
+        mixed32 = {"ms": round(float(np.median(msw)), 3), "outer": winfo.outer, "inner_total": winfo.inner_total,
+                   "true_res": (op.residual(b, xm)[1] / bb) ** 0.5,
+                   "speedup_vs_fp64": round(float(np.median(ms64) / np.median(msw)), 3)}
     th, visible = cpu_threads()
-    log(f"graph: fp64 {np.median(ms64):.2f} ms ({info.executed} it), mixed {np.median(msm):.2f} ms; oracle JPCG on "
-        f"{th} host threads ...")
+    log(f"graph: fp64 {np.median(ms64):.2f} ms ({info.executed} it), mixed {np.median(msm):.2f} ms, with fp32 "
+        f"work vectors {mixed32['ms'] if mixed32 else None} ms; oracle JPCG on {th} host threads ...")
     cpu_s, ref = cpu_jpcg(A, sysm.b, sysm.max_iter, sysm.tol, th, 1)
     xr = ref["x"]
     nr = np.linalg.norm(xr)
@@ -308,8 +319,10 @@ def bench_graph(args):
            "fp64": {"ms": round(float(np.median(ms64)), 3), "iterations": info.executed, "true_res": true64,
                     "rel_err_vs_oracle": float(np.linalg.norm(x.cpu().numpy() - xr) / nr)},
            "mixed": {"ms": round(float(np.median(msm)), 3), "outer": minfo.outer, "inner_total": minfo.inner_total,
-                     "true_res": truem, "rel_err_vs_oracle": float(np.linalg.norm(xm.cpu().numpy() - xr) / nr)},
+                     "true_res": truem, "rel_err_vs_oracle": float(np.linalg.norm(xmh - xr) / nr)},
            "speedup_mixed_vs_fp64": round(float(np.median(ms64) / np.median(msm)), 3),
+           "mixed_fp32_vectors": dict(mixed32, rel_err_vs_oracle=float(np.linalg.norm(xm.cpu().numpy() - xr) / nr))
+           if mixed32 else None,
            "cpu_oracle_jpcg": {"iterations": ref["executed"], "ms_all": round(cpu_s * 1e3, 1), "threads": th,
                                "logical_cpus_visible": visible}}
     out["gpu_speedup_vs_cpu_all"] = round(cpu_s * 1e3 / out["fp64"]["ms"], 1)

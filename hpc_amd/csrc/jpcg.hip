@@ -516,9 +516,9 @@ static_assert(kPanelChunk % kPanelBlock == 0, "whole chunk entries per lane");
 
 // acc: LDS of panel_lds_bytes(A.block_rows) bytes; on return acc[i] holds
 // y[b*block_rows + i] for the block's rows (after a __syncthreads).
-template <typename VT>
+template <typename VT, typename XT = double>
 __device__ __forceinline__ void panel_rows(const PanelDev& A, int n, int b, const double* __restrict__ dsplit,
-                                           const double* __restrict__ x, double* acc) {
+                                           const XT* __restrict__ x, double* acc) {
 #pragma clang fp contract(off)
     constexpr int J = kPanelChunk / kPanelBlock;
     double* prod = acc + A.block_rows;
@@ -526,7 +526,7 @@ __device__ __forceinline__ void panel_rows(const PanelDev& A, int n, int b, cons
     const int tid = threadIdx.x;
     const int row0 = b * A.block_rows;
     const int rows = min(A.block_rows, n - row0);
-    for (int i = tid; i < rows; i += kPanelBlock) acc[i] = dsplit ? dsplit[row0 + i] * x[row0 + i] : 0.0;
+    for (int i = tid; i < rows; i += kPanelBlock) acc[i] = dsplit ? dsplit[row0 + i] * (double)x[row0 + i] : 0.0;
     __syncthreads();
     const int* seg = A.seg + (int64_t)b * A.P;
     const VT* __restrict__ val = static_cast<const VT*>(A.val);
@@ -534,7 +534,7 @@ __device__ __forceinline__ void panel_rows(const PanelDev& A, int n, int b, cons
     const uint16_t* __restrict__ lrow = A.lrow;
     for (int p = 0; p < A.P; ++p) {
         const int s1 = seg[p + 1];
-        const double* __restrict__ xp = x + (int64_t)p * kPanelCols;
+        const XT* __restrict__ xp = x + (int64_t)p * kPanelCols;
         for (int c0 = seg[p]; c0 < s1; c0 += kPanelChunk) {
             const int cn = min(kPanelChunk, s1 - c0);
             // lanes past the chunk's end re-read its first entry (in bounds)
@@ -549,7 +549,7 @@ __device__ __forceinline__ void panel_rows(const PanelDev& A, int n, int b, cons
                 lr[k] = k < cn ? lrow[e] : (uint16_t)0xFFFF;
             }
 #pragma unroll
-            for (int j = 0; j < J; ++j) prod[tid + j * kPanelBlock] = (double)v[j] * xp[o[j]];
+            for (int j = 0; j < J; ++j) prod[tid + j * kPanelBlock] = (double)v[j] * (double)xp[o[j]];
             __syncthreads();
 #pragma unroll
             for (int j = 0; j < J; ++j) {
@@ -581,21 +581,22 @@ __global__ __launch_bounds__(kPanelBlock) void spmv_panels(PanelDev A, int n, co
 }
 
 // CG-loop SpMV on the panel layout: Ap = A p and block blk's p.Ap partial in partA[blk]
-template <typename VT>
+template <typename VT, typename PT>
 __global__ __launch_bounds__(kPanelBlock) void cg_spmv_panels(CgParams P) {
     extern __shared__ double plds[];
     __shared__ double red[kPanelBlock / 64];
     CgState* st = P.st;
     if (st->stop) return;
     const int blk = blockIdx.x;
-    const double* __restrict__ p = P.p;
-    panel_rows<VT>(P.pan, P.n, blk, P.dsplit, p, plds);
+    const PT* __restrict__ p = wv<PT>(P.p);
+    PT* __restrict__ Ap = wv<PT>(P.Ap);
+    panel_rows<VT, PT>(P.pan, P.n, blk, P.dsplit, p, plds);
     const int row0 = blk * P.pan.block_rows, rows = min(P.pan.block_rows, P.n - row0);
     double pap = 0.0;
     for (int i = threadIdx.x; i < rows; i += kPanelBlock) {
-        const double y = plds[i];
-        P.Ap[row0 + i] = y;
-        pap += p[row0 + i] * y;
+        const PT y = (PT)plds[i];  // with fp32 work vectors the partial uses the stored Ap
+        Ap[row0 + i] = y;
+        pap += (double)p[row0 + i] * (double)y;
     }
     pap = block_sum_w<kPanelBlock / 64>(pap, red);
     if (threadIdx.x == 0) P.partA[blk] = pap;
@@ -1106,7 +1107,7 @@ hipError_t launch_jpcg(const JpcgParams& P0, bool f32, bool c16, int G, hipStrea
 }
 
 hipError_t launch_cg_setup(const CgParams& P, hipStream_t s) {
-    if (P.w32 && !P.dia_on) return hipErrorInvalidValue;
+    if (P.w32 && !P.dia_on && !P.pan_on) return hipErrorInvalidValue;
     if (P.w32)
         cg_setup<float><<<P.G2, kBlock, 0, s>>>(P);
     else
@@ -1118,10 +1119,17 @@ hipError_t launch_cg_setup(const CgParams& P, hipStream_t s) {
 hipError_t launch_cg_iteration(const CgParams& P, bool f32, bool c16, hipStream_t s) {
     if (P.pan_on) {
         const size_t lds = (size_t)panel_lds_bytes(P.pan.block_rows);
-        if (f32)
-            cg_spmv_panels<float><<<P.pan.nb, kPanelBlock, lds, s>>>(P);
-        else
-            cg_spmv_panels<double><<<P.pan.nb, kPanelBlock, lds, s>>>(P);
+        if (P.w32) {
+            if (f32)
+                cg_spmv_panels<float, float><<<P.pan.nb, kPanelBlock, lds, s>>>(P);
+            else
+                cg_spmv_panels<double, float><<<P.pan.nb, kPanelBlock, lds, s>>>(P);
+        } else {
+            if (f32)
+                cg_spmv_panels<float, double><<<P.pan.nb, kPanelBlock, lds, s>>>(P);
+            else
+                cg_spmv_panels<double, double><<<P.pan.nb, kPanelBlock, lds, s>>>(P);
+        }
     } else if (P.dia_on) {
         const int g = dia_groups(P.n, f32);
         if (P.w32) {
@@ -1147,7 +1155,7 @@ hipError_t launch_cg_iteration(const CgParams& P, bool f32, bool c16, hipStream_
 }
 
 hipError_t launch_cg_vector_steps(const CgParams& P, hipStream_t s) {
-    if (P.w32 && !P.dia_on) return hipErrorInvalidValue;
+    if (P.w32 && !P.dia_on && !P.pan_on) return hipErrorInvalidValue;
     if (P.split_alpha) cg_alpha<<<P.nA2, kBlock, 0, s>>>(P);
     if (P.w32) {
         cg_update_r<float><<<P.G2, kBlock, 0, s>>>(P);

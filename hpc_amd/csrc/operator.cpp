@@ -700,8 +700,8 @@ CgParams Operator::cg_params(const double* diag, const double* b, double* x, uin
 
 void Operator::set_vector_bits(int bits) {
     if (bits != 32 && bits != 64) throw invalid("vector bits must be 32 or 64.");
-    if (bits == 32 && solve_layout_ != kDiagonals)
-        throw invalid("fp32 work vectors need the diagonal solve layout.");
+    if (bits == 32 && solve_layout_ != kDiagonals && solve_layout_ != kPanels)
+        throw invalid("fp32 work vectors need the diagonal or the panel solve layout.");
     w32_ = bits == 32;
     drop_graph();
 }

@@ -171,11 +171,13 @@ class Operator {
 
     // Work vectors of the solves: 64 (fp64) or 32 (r, p, Ap and the Jacobi
     // factors stored in fp32; x, the arithmetic and the dot products stay
-    // fp64).  32 needs the
-    // diagonal solve layout and runs the multi-kernel loop; while another
-    // layout is in use the solves run with fp64 vectors.
+    // fp64).  32 needs the diagonal or the panel solve layout and runs the
+    // multi-kernel loop; while another layout is in use the solves run with
+    // fp64 vectors.
     void set_vector_bits(int bits);
-    int vector_bits() const { return w32_ && solve_layout_ == kDiagonals ? 32 : 64; }
+    int vector_bits() const {
+        return w32_ && (solve_layout_ == kDiagonals || solve_layout_ == kPanels) ? 32 : 64;
+    }
 
   private:
     SolveResult solve_persistent(const double* diag, const double* b, double* x, uint32_t max_iter, double tol,

@@ -141,7 +141,7 @@ class CsrOperator:
 
     def set_vectors(self, bits: int):
         """Work vectors of the solves: 64, or 32 to store r, p, A p and the Jacobi factors in fp32 (x, the
-        arithmetic and the dot products stay fp64).  32 needs the diagonal solve layout."""
+        arithmetic and the dot products stay fp64).  32 needs the diagonal or the panel solve layout."""
         check_dev(lib().xjpcg_op_set_vectors(self._h, int(bits), None))
 
     @property

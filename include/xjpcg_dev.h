@@ -92,8 +92,9 @@ XJPCG_Status_t xjpcg_op_solve_layout(const xjpcg_op_t* op, int* h_layout);
  * stores r, p, A p and the Jacobi factors 1/diag in fp32 while x, the
  * arithmetic and the dot products stay fp64 (meant for the inner operator of
  * xjpcg_solve_mixed; halves the vector traffic of an iteration).  32 needs the
- * diagonal solve layout (INVALID_VALUE otherwise) and runs the multi-kernel
- * loop; while another solve layout is set later, the solves use fp64 vectors.
+ * diagonal or the panel solve layout (INVALID_VALUE otherwise) and runs the
+ * multi-kernel loop; while another solve layout is set later, the solves use
+ * fp64 vectors.
  * bits = 0 queries.  *h_bits (may be NULL) receives the bits in effect.
  * Extension of the device API: the reference's solver keeps all vectors in
  * fp64 (pcg/sw/include/pcgImp.hpp:86-120). */

@@ -3,7 +3,8 @@
 * 27-point stencil 160^3 (n = 4,096,000, 109M stored entries), tol 1e-10: the fp64 solve on the layout the
   operator picks for its solves, and the mixed-refined solve with fp64 and with fp32 work vectors in the
   inner operator, against the oracle's JPCG on the host cores.
-* Graph Laplacian n = 4,000,000 (84.4M stored entries, lognormal degrees capped at 2000), tol 1e-10: the same.
+* Graph Laplacian n = 4,000,000 (84.4M stored entries, lognormal degrees capped at 2000), tol 1e-10: the same,
+  fp32 work vectors on the panel layout included.
 * 512 anisotropic diffusion systems x 16 right-hand sides (N = 44.1M rows, 266M entries), tol 1e-8: every
   pair's true residual from the oracle's SpMV, and oracle solves of sampled pairs that include the largest
   system.
@@ -104,7 +105,7 @@ def test_stencil27_160_fp64_and_mixed_vs_oracle(cuda):
 
 
 def test_graph4m_fp64_and_mixed_vs_oracle(cuda):
-    _fp64_and_mixed(cuda, workloads.c3_graph(4_000_000), "graph n=4M")
+    _fp64_and_mixed(cuda, workloads.c3_graph(4_000_000), "graph n=4M", fp32_vectors=True)
 
 
 def test_batch512x16_all_pairs_and_sampled_oracle(cuda):

@@ -122,6 +122,24 @@ def test_panel_split_fp32_and_mixed(cuda):
     true = np.linalg.norm(b - A.to_scipy() @ xm.cpu().numpy()) / np.linalg.norm(b)
     _say(f"mixed on panels: outer {info.outer} inner {info.inner_total} true res {true:.2e}")
     assert true < 1e-10
+    # fp32 work vectors on the panels: the refined solve still reaches 1e-10 and agrees with the fp64
+    # oracle's x; a plain fp32-vector solve to 1e-6 stays within 2% of the oracle's iteration count
+    inner.set_vectors(32)
+    assert inner.vector_bits == 32
+    xw, iw = device.solve_mixed(outer, inner, _t(cuda, A.diag), _t(cuda, b), 1e-10)
+    xwh = xw.cpu().numpy()
+    truew = np.linalg.norm(b - A.to_scipy() @ xwh) / np.linalg.norm(b)
+    o = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, b, 20000, 1e-10, threads=8)
+    errw = np.linalg.norm(xwh - o["x"]) / np.linalg.norm(o["x"])
+    _say(f"mixed on panels, fp32 vectors: outer {iw.outer} inner {iw.inner_total} true res {truew:.2e} "
+         f"vs oracle {errw:.2e}")
+    assert truew < 1e-10 and errw < 1e-8
+    x6, i6 = inner.solve(_t(cuda, A.diag), _t(cuda, b), 20000, 1e-6)
+    o6 = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, b, 20000, 1e-6, threads=8)
+    t6 = np.linalg.norm(b - A.to_scipy() @ x6.cpu().numpy()) / np.linalg.norm(b)
+    _say(f"panels, fp32 vectors to 1e-6: {i6.executed} iterations (oracle {o6['executed']}), true res {t6:.2e}")
+    assert abs(i6.executed - o6["executed"]) <= max(2, o6["executed"] // 50)
+    assert i6.residual < 1e-6 and t6 < 1.5e-6
 
 
 @pytest.mark.parametrize("kind", ["2d5", "3d27"])
@@ -446,6 +464,40 @@ def test_mixed_inner_fp32_vectors_on_diagonals(cuda):
     outer.set_vectors(64)
     x64, i64 = outer.solve(diag, bt, 20000, 1e-6)
     assert i64.executed == o6["executed"] or abs(i64.executed - o6["executed"]) <= 1
+
+
+def test_fp32_vectors_edge_cases(cuda):
+    """fp32 work vectors on the diagonal layout keep the loop's semantics: b = 0 gives x = 0 and no
+    iteration; the iteration cap runs maxIter+1 rounds and reports maxIter, like the fp64 loop and the
+    oracle; the records of a capped solve count 0..K-1 with the recursive residual of each round, within
+    fp32 rounding of the fp64 loop's early records."""
+    import torch
+    from hpc_amd import device
+    A = workloads.stencil(workloads.STENCIL_3D7, 91, 81, 81, [0.5, 1.5, 2.5])
+    b = workloads.uniform(6, 1, A.n, -1, 1)
+    op = device.CsrOperator(_t(cuda, A.rowptr), _t(cuda, A.col), _t(cuda, A.val), rowptr_host=A.rowptr)
+    op.set_layout(4)
+    d, bt = _t(cuda, A.diag), _t(cuda, b)
+    cap = 12
+    rec64 = torch.zeros((cap + 2) * 64, dtype=torch.uint8, device=cuda)
+    x64, i64 = op.solve(d, bt, cap, 1e-12, records=rec64)
+    op.set_vectors(32)
+    xz, iz = op.solve(d, _t(cuda, np.zeros(A.n)), cap, 1e-12)
+    assert iz.executed == 0 and iz.iterations == 0 and iz.residual == 0.0
+    assert not xz.cpu().numpy().any()
+    rec32 = torch.zeros((cap + 2) * 64, dtype=torch.uint8, device=cuda)
+    x32, i32 = op.solve(d, bt, cap, 1e-12, records=rec32)
+    o = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, b, cap, 1e-12)
+    assert i32.executed == i64.executed == o["executed"] == cap + 1
+    assert i32.iterations == i64.iterations == o["iter"] == cap
+    r32 = rec32.cpu().numpy().view(oracle.RECORD_DTYPE)
+    r64 = rec64.cpu().numpy().view(oracle.RECORD_DTYPE)
+    assert np.array_equal(r32["iter"][1:cap + 2], np.arange(cap + 1))
+    assert np.allclose(r32["res"][1:cap + 2], r64["res"][1:cap + 2], rtol=1e-4)
+    assert np.allclose(r32["alpha"][1:cap + 2], r64["alpha"][1:cap + 2], rtol=1e-4)
+    _say(f"fp32 vectors capped at {cap}: res {i32.residual:.6e} vs fp64 {i64.residual:.6e}; "
+         f"x rel diff {np.linalg.norm(x32.cpu().numpy() - x64.cpu().numpy()) / np.linalg.norm(x64.cpu().numpy()):.2e}")
+    assert abs(i32.residual - i64.residual) <= 1e-4 * i64.residual
 
 
 def test_solve_layout_chosen_separately(cuda):
