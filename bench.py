@@ -434,6 +434,44 @@ def bench_batched(args):
     return out
 
 
+def bench_shard(args):
+    """One GPU's share of the 160^3 27-point stencil split by contiguous row blocks over P = 2, 4, 8 GPUs: a
+    160 x 160 x (160/P) slab (Dirichlet faces where the neighbours would be), the SpMV on the layout the operator
+    picks, timed on its own bytes, and the JPCG iteration (graph-replayed multi-kernel loop, 200 iterations);
+    beside them the p halo a rank would exchange per iteration (two 160^2 planes, fp64) and its time on one
+    xGMI link at 153 GB/s.  The transport and the two all-reduces per iteration are not measured."""
+    import torch
+    from hpc_amd import device, workloads
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    m = args.grid
+    out = {"workload": f"row-block share of the 27-point stencil {m}^3 for P GPUs: {m} x {m} x {m}/P slab",
+           "shares": {}}
+    for P in (2, 4, 8):
+        nz = m // P
+        A = workloads.stencil(workloads.STENCIL_3D27, m, m, nz, [26.0, -1.0])
+        op = device.CsrOperator(t(A.rowptr), t(A.col), t(A.val), rowptr_host=A.rowptr)
+        ms = spmv_launches(op, A.n, stream, args.spmv_repeats)
+        nb = op.spmv_bytes()
+        mean_us = float(ms.mean()) * 1e3
+        diag = t(A.diag)
+        b = t(np.ones(A.n))
+        x = torch.empty_like(b)
+        it_ms, _ = _timed(lambda: op.solve(diag, b, 199, 1e-300, x=x), 3, 1, stream)
+        halo = 2 * m * m * 8
+        out["shares"][f"P{P}"] = {
+            "rows": A.n, "nnz": A.nnz, "layout": LAYOUT_NAMES[op.layout],
+            "spmv_us": round(mean_us, 2), "spmv_bytes": nb, "spmv_GBs": round(nb / (mean_us * 1e-6) / 1e9, 1),
+            "spmv_frac_of_peak": round(nb / (mean_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+            "jpcg_us_per_iteration": round(float(np.median(it_ms)) * 1e3 / 200, 2),
+            "halo_bytes_per_iteration": halo, "halo_us_one_xgmi_link": round(halo / 153e9 * 1e6, 2)}
+        log(f"shard P={P}: {out['shares'][f'P{P}']}")
+        del op
+        torch.cuda.empty_cache()
+    return out
+
+
 def bench_autograd(args):
     """Differentiable solve on 2D variable-coefficient diffusion 512^2: forward JPCG, then backward
     (adjoint JPCG + SDDMM) for L = 0.5 ||x - x_target||^2, fp64, tol 1e-12."""
@@ -836,7 +874,7 @@ def main():
     ap.add_argument("--tune", action="store_true", help="autotune the batched solves first")
     ap.add_argument("--batch-steps", type=int, default=1, help="timed batched solves (each ~10 s)")
     ap.add_argument("--mode", choices=["full", "spmv-profile", "laplace2d", "graph", "batched", "autograd",
-                                       "inexact-mixed", "batched-profile", "ingest", "signature"],
+                                       "inexact-mixed", "batched-profile", "ingest", "signature", "shard"],
                     default="full",
                     help="spmv-profile: one 256 MiB device copy (counter calibration) then 10 SpMV launches, "
                          "for rocprofv3 --pmc runs; laplace2d / graph / batched / autograd / inexact-mixed: one "
@@ -864,7 +902,7 @@ def main():
         return
     single = {"laplace2d": bench_laplace2d, "graph": bench_graph, "batched": bench_batched,
               "autograd": bench_autograd, "inexact-mixed": bench_inexact_mixed, "ingest": bench_ingest,
-              "signature": bench_signature}
+              "signature": bench_signature, "shard": bench_shard}
     if args.mode in single:
         print(json.dumps(single[args.mode](args)), flush=True)
         return
@@ -873,7 +911,7 @@ def main():
         extra = {}
         for name, fn in (("laplace2d_256", bench_laplace2d), ("graph_4m", bench_graph),
                          ("batch_512x16", bench_batched), ("autograd_512", bench_autograd),
-                         ("inexact_fp32_512", bench_inexact_mixed)):
+                         ("inexact_fp32_512", bench_inexact_mixed), ("shard_160", bench_shard)):
             t0 = time.time()
             try:
                 extra[name] = fn(args)
