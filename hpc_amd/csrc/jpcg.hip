@@ -792,9 +792,9 @@ __global__ __launch_bounds__(kBlock) void spmv_dia(int n, DiaDev D, const double
 
 // CG SpMV on the diagonal layout: 256 R consecutive rows per workgroup, R per
 // lane (every address follows from the block index, so nothing waits on a
-// table); Ap is stored and each wavefront leaves one p.Ap partial, a lane's
-// rows' products added in row order, then an xor-butterfly within the
-// wavefront -- a fixed order, no barrier.  With fp32 work vectors (PT =
+// table); Ap is stored and each workgroup leaves one p.Ap partial, a lane's
+// rows' products added in row order, an xor-butterfly within each wavefront,
+// then the wavefronts in order -- a fixed order.  With fp32 work vectors (PT =
 // float) p is widened as it is read, Ap is rounded as it is stored and the
 // partial uses the stored value.
 template <typename VT, typename PT>
@@ -826,8 +826,12 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_dia(CgParams P) {
                 pap += pi[r] * (double)a;
             }
     }
-    pap = wave_sum(pap);
-    if ((threadIdx.x & 63) == 0) P.partA[b * kWaves + (threadIdx.x >> 6)] = pap;
+    // one partial per workgroup rather than per wavefront: a quarter of the
+    // partials for cg_alpha or the vector kernels to fold (160^3 mixed solve
+    // with fp32 work vectors 43.7 ms against 44.0 ms, same box)
+    __shared__ double red[kWaves];
+    pap = block_sum(pap, red);
+    if (threadIdx.x == 0) P.partA[b] = pap;
     // k advances after the last load: a store ahead of the loads could alias
     // the tables and would turn their scalar loads into vector loads
     if (blockIdx.x == 0 && threadIdx.x == 0) st->k = st->k + 1;  // no other workgroup of this launch reads k

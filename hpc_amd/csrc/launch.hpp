@@ -42,12 +42,12 @@ constexpr int dia_rows_per_lane(bool f32) { return f32 ? 1 : 2; }
 // load; the padding row's mask is 0 and its values are 0.
 constexpr int dia_ld(int n) { return (n + 1) & ~1; }
 // workgroups of the diagonal kernels (kDiaBlock lanes each) and the CG
-// SpMV's p.Ap partials (one per wavefront)
+// SpMV's p.Ap partials (one per workgroup)
 constexpr int kDiaBlock = 256;
 constexpr int dia_groups(int n, bool f32) {
     return (n + kDiaBlock * dia_rows_per_lane(f32) - 1) / (kDiaBlock * dia_rows_per_lane(f32));
 }
-constexpr int dia_cg_partials(int n, bool f32) { return dia_groups(n, f32) * (kDiaBlock / 64); }
+constexpr int dia_cg_partials(int n, bool f32) { return dia_groups(n, f32); }
 // Workgroups of the CG vector kernels (256 lanes, grid-stride; one r.r / r.z
 // partial each, and every workgroup of the x/p update re-sums all of them):
 // one row per lane up to 256 workgroups, so a small system's update is one
@@ -131,8 +131,8 @@ struct CgState {
 };
 
 // Arguments of the multi-kernel loop: SpMV (one workgroup per tile or per row
-// block of the panel layout, with one p.Ap partial each; one partial per
-// wavefront of 256-row workgroups on the diagonal layout), optionally a
+// block of the panel layout or workgroup of the diagonal layout, with one
+// p.Ap partial each), optionally a
 // reduction of those partials, then two vector kernels (G2 workgroups,
 // grid-stride).  ntiles counts the SpMV workgroups, hence the partials.
 struct CgParams {

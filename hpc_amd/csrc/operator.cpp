@@ -666,8 +666,8 @@ CgParams Operator::cg_params(const double* diag, const double* b, double* x, uin
     P.dia_on = solve_layout_ == kDiagonals ? 1 : 0;
     P.dia = A_.dia;
     P.tiles = A_.ntiles;
-    // p.Ap partials: one per panel row block, per wavefront of the diagonal
-    // layout's workgroups, or per row tile
+    // p.Ap partials: one per panel row block, per workgroup of the diagonal
+    // layout, or per row tile
     P.ntiles = P.pan_on ? A_.pan.nb : P.dia_on ? dia_cg_partials(A_.n, A_.val_f32) : A_.ntiles;
     P.nnz = (int)A_.nnz;
     P.G2 = G2_;
