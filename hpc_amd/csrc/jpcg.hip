@@ -225,6 +225,10 @@ __global__ __launch_bounds__(kBlock) void cg_setup_finish(CgParams P) {
         st->k = 0xFFFFFFFFu;  // the first cg_spmv advances it to 0
         st->it = 0;
         st->stop = dot > 0.0 ? 0 : 1;  // b == 0: x = 0, no iteration
+        if (P.host_stop && !(dot > 0.0)) {
+            *P.host_stop = 1;
+            __threadfence_system();
+        }
         st->stop_k2 = 0;
         st->K = 0;
         if (P.records) {
@@ -459,7 +463,13 @@ __global__ __launch_bounds__(kBlock) void cg_update_xp(CgParams P) {
             more = it < P.max_rounds;
         }
         st->it = it;
-        if (!more) st->stop = 1;
+        if (!more) {
+            st->stop = 1;
+            if (P.host_stop) {
+                *P.host_stop = 1;
+                __threadfence_system();
+            }
+        }
     }
 }
 

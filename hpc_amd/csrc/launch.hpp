@@ -163,6 +163,10 @@ struct CgParams {
     Record* records = nullptr;
     double tol = 0.0;
     uint32_t max_rounds = 0;
+    // pinned host word the kernel that ends the loop also sets to 1 (may be
+    // null): the host reads it after a chunk's completion event instead of
+    // copying the device flag, which would put a copy between chunks
+    int* host_stop = nullptr;
 };
 
 // tiles above which the SpMV's p.Ap partials are first folded by cg_alpha

@@ -735,7 +735,13 @@ void Operator::build_graph(const CgParams& P) {
 
 SolveResult Operator::solve_multi(const double* diag, const double* b, double* x, uint32_t max_iter, double tol,
                                   void* records, hipStream_t s) {
-    const CgParams P = cg_params(diag, b, x, max_iter, tol, records);
+    CgParams P = cg_params(diag, b, x, max_iter, tol, records);
+    // the loop's end is signalled to the host in pinned memory (no copy in the
+    // stream between chunks); the previous solve has finished, so the word
+    // can be cleared from the host
+    volatile int* host_stop = host_flags_;
+    *host_stop = 0;
+    P.host_stop = host_flags_;
     // the work stream follows everything already queued on the caller's stream
     hip_check(hipEventRecord(ev_join_, s), "event record");
     hip_check(hipStreamWaitEvent(work_, ev_join_, 0), "stream wait");
@@ -745,12 +751,13 @@ SolveResult Operator::solve_multi(const double* diag, const double* b, double* x
     // graph captured for (x, records, max_rounds, values) replays for any b.
     if (graph_ok_ && (!gexec_ || gkey_.x != P.x || gkey_.records != P.records || gkey_.max_rounds != P.max_rounds ||
                       gkey_.val != P.val || gkey_.col != P.col || gkey_.pan_on != P.pan_on ||
-                      gkey_.pat_on != P.pat_on || gkey_.dia_on != P.dia_on || gkey_.w32 != P.w32))
+                      gkey_.pat_on != P.pat_on || gkey_.dia_on != P.dia_on || gkey_.w32 != P.w32 ||
+                      gkey_.host_stop != P.host_stop))
         build_graph(P);
-    // Enqueue iterations in chunks; after each chunk the stop flag is copied to
-    // pinned memory.  The host checks chunk c-1's flag while chunk c is queued,
-    // so the GPU never idles and at most one chunk of launches returns at once
-    // after convergence.
+    // Enqueue iterations in chunks, with an event after each.  The host waits
+    // for chunk c-1's event while chunk c is queued and then reads the pinned
+    // stop word, so the GPU never idles and at most one chunk of launches
+    // returns at once after convergence.
     uint64_t enq = 0;
     int slot = 0;
     bool pending = false;
@@ -765,12 +772,10 @@ SolveResult Operator::solve_multi(const double* diag, const double* b, double* x
                 hip_check(launch_cg_iteration(P, A_.val_f32, A_.col16, work_), "CG iteration launch");
             enq += c;
         }
-        hip_check(hipMemcpyAsync(&host_flags_[slot], &state_.ptr->stop, sizeof(int), hipMemcpyDeviceToHost, work_),
-                  "read stop flag");
         hip_check(hipEventRecord(ev_[slot], work_), "event record");
         if (pending) {
             hip_check(hipEventSynchronize(ev_[slot ^ 1]), "event sync");
-            if (host_flags_[slot ^ 1]) break;
+            if (*host_stop) break;
         }
         pending = true;
         slot ^= 1;
