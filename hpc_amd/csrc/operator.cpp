@@ -199,6 +199,7 @@ Operator::Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t*
     A_.tile_k = tilek_.ptr;
     A_.ntiles = (int)t.size() - 1;
     G2_ = vector_groups(n);
+    chunk_ = chunk_for(n);
     partA_.ensure(std::max(1, A_.ntiles));
     partA2_.ensure(kAlphaBlocks);
     partB_.ensure(G2_);
@@ -720,7 +721,7 @@ void Operator::build_graph(const CgParams& P) {
         return;
     }
     hipError_t e = hipSuccess;
-    for (int c = 0; c < kChunk && e == hipSuccess; ++c) e = launch_cg_iteration(P, A_.val_f32, A_.col16, work_);
+    for (int c = 0; c < chunk_ && e == hipSuccess; ++c) e = launch_cg_iteration(P, A_.val_f32, A_.col16, work_);
     hipError_t e2 = hipStreamEndCapture(work_, &g);
     if (e == hipSuccess && e2 == hipSuccess && g) e = hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0);
     if (g) (void)hipGraphDestroy(g);
@@ -763,11 +764,11 @@ SolveResult Operator::solve_multi(const double* diag, const double* b, double* x
     bool pending = false;
     while (enq < P.max_rounds) {
         const uint64_t left = P.max_rounds - enq;
-        if (gexec_ && left >= (uint64_t)kChunk) {
+        if (gexec_ && left >= (uint64_t)chunk_) {
             hip_check(hipGraphLaunch(gexec_, work_), "graph launch");
-            enq += kChunk;
+            enq += chunk_;
         } else {
-            const uint32_t c = (uint32_t)std::min<uint64_t>(left, kChunk);
+            const uint32_t c = (uint32_t)std::min<uint64_t>(left, chunk_);
             for (uint32_t j = 0; j < c; ++j)
                 hip_check(launch_cg_iteration(P, A_.val_f32, A_.col16, work_), "CG iteration launch");
             enq += c;

@@ -203,7 +203,13 @@ class Operator {
 
     const void* col_stream() const { return A_.col16 ? (const void*)A_.col16 : (const void*)A_.col; }
 
-    static constexpr int kChunk = 8;  // iterations per captured graph
+    // Iterations per captured graph: a chunk boundary costs a graph launch and
+    // an event, and after convergence up to a chunk of launches return at
+    // once.  Measured on one box: 32 iterations take the 256^2 Laplacian from
+    // 9.89 to 9.27 ms and cost the 160^3 refined solve 43.9 -> 44.5 ms, so
+    // operators up to 2^20 rows use 32, larger ones 8.
+    static constexpr int chunk_for(int n) { return n <= (1 << 20) ? 32 : 8; }
+    int chunk_ = 8;
     // panel layout considered from this many rows (x = 2 MiB, beyond what the
     // tile kernel's gathers keep in L2 once columns scatter)
     static constexpr int32_t kPanelAutoRows = 1 << 18;
