@@ -252,7 +252,7 @@ Operator::Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t*
     // kPanelAutoRows rows, columns ascending in panel order), else the tiles
     // or the pattern tiles (whichever SpMV is faster; both give the same
     // per-tile p.Ap partials, so the same iterates).  The diagonal and panel
-    // layouts group the p.Ap partials their own way (per wavefront, per row
+    // layouts group the p.Ap partials their own way (per workgroup, per row
     // block).
     if (nnz > 0) {
         const float t_tiles = time_layout(kTiles, s);

@@ -143,6 +143,17 @@ def record_encode(rec: dict) -> bytes:
     return out.tobytes()
 
 
+def record_decode(raw: bytes) -> dict:
+    """One 64-byte record as a dict of its fields (CGSolverInstr::load and decode,
+    pcg/hw/include/impl/cgInstr.hpp:94-97 and 133-148)."""
+    inp = np.frombuffer(bytes(raw), np.uint8)
+    if inp.size != 64:
+        raise ValueError("a record is 64 bytes")
+    a = np.zeros(1, RECORD_DTYPE)
+    lib().orc_record_decode(inp.ctypes.data, a.ctypes.data)
+    return {k: a[k][0].item() for k in RECORD_DTYPE.names}
+
+
 def jpcg(rowptr, col, val, diag, b, max_iter, tol, threads=1, records=False):
     """Returns dict(x, iter, res, executed, rr_hist, records)."""
     n = rowptr.size - 1

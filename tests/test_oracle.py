@@ -176,14 +176,23 @@ def test_record_codec_matches_reference_bytes():
     R.ref_record_encode(3, 17, 4096, 5001, 1.5e-20, 2.5e-9, -3.25, 0.125, 7.0e-3, 0x0123456789ABCDEF,
                         out.ctypes.data)
     assert oracle.record_encode(rec) == out.tobytes()
+    # the oracle's decoder and the reference's read the same fields back
+    assert oracle.record_decode(out.tobytes()) == rec
+    u, f, clk = np.zeros(4, np.uint32), np.zeros(5), np.zeros(1, np.uint64)
+    R.ref_record_decode(out.ctypes.data, u.ctypes.data, f.ctypes.data, clk.ctypes.data)
+    assert [int(v) for v in u] == [3, 17, 4096, 5001]
+    assert f.tolist() == [1.5e-20, 2.5e-9, -3.25, 0.125, 7.0e-3] and int(clk[0]) == 0x0123456789ABCDEF
     # record 0 as the reference host programs it (maxIter+1, Tols = dot tol^2)
     R.ref_record0.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                               ctypes.c_void_p]
     r0 = np.zeros(64, np.uint8)
     R.ref_record0(100, 64, 2.0, 1e-3, 0.5, r0.ctypes.data)
-    dec = r0.view(oracle.RECORD_DTYPE)[0]
+    dec = oracle.record_decode(r0.tobytes())
+    assert dec == {k: v.item() for k, v in zip(oracle.RECORD_DTYPE.names, r0.view(oracle.RECORD_DTYPE)[0])}
     assert dec["max_iter"] == 101 and dec["vec_size"] == 64 and dec["res"] == 2.0 and dec["rz"] == 0.5
     assert dec["tols"] == 2.0 * 1e-3 * 1e-3
+    with pytest.raises(ValueError):
+        oracle.record_decode(b"\0" * 63)
 
 
 # --------------------------------------------------------------------------- JPCG
