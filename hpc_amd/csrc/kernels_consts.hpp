@@ -22,7 +22,15 @@ constexpr int kPanelColsHost = 65536;
 constexpr int kPanelRowsMaxHost = 4096;
 constexpr int kPanelBlockHost = 1024;
 constexpr int kPanelChunkHost = 2048;
-// LDS bytes of one panel workgroup: row sums + staged products + staged row offsets
+// An entry's uint16 row word: the row offset in its block in the low
+// kPanelRowBits bits; above them, for the first entry of a row's run inside a
+// staged chunk, the run's length (kPanelRunScan: that many or more, the
+// kernel scans for the end), 0 for the run's other entries.
+constexpr int kPanelRowBits = 12;
+constexpr int kPanelRunScan = 15;
+static_assert(kPanelRowsMaxHost <= (1 << kPanelRowBits) && kPanelRunScan < (1 << (16 - kPanelRowBits)),
+              "row offset and run code share 16 bits");
+// LDS bytes of one panel workgroup: row sums + staged products + staged row words
 constexpr int panel_lds_bytes(int block_rows) { return block_rows * 8 + kPanelChunkHost * 10; }
 // column-offset pattern tiles run two tiles per workgroup; the CG loop keeps
 // one p.Ap partial per workgroup

@@ -155,6 +155,23 @@ PanelPlan plan_panels(int32_t n, const int32_t* rp, const int32_t* col, int bloc
             pl.perm[(size_t)q] = k;
         }
     }
+    // run codes: the kernels stage a segment kPanelChunkHost entries at a
+    // time; the first entry of each row's run inside such a chunk carries the
+    // run's length (capped at kPanelRunScan, "scan for the end") in the bits
+    // above the row offset, the other entries carry 0
+    for (int64_t q = 0; q < nseg; ++q) {
+        const int64_t s1 = pl.seg[(size_t)q + 1];
+        for (int64_t c0 = pl.seg[(size_t)q]; c0 < s1; c0 += kPanelChunkHost) {
+            const int64_t c1 = std::min<int64_t>(c0 + kPanelChunkHost, s1);
+            for (int64_t k = c0; k < c1;) {
+                int64_t e = k + 1;
+                while (e < c1 && pl.lrow[(size_t)e] == pl.lrow[(size_t)k]) ++e;
+                const int64_t len = std::min<int64_t>(e - k, kPanelRunScan);
+                pl.lrow[(size_t)k] = (uint16_t)(pl.lrow[(size_t)k] | (len << kPanelRowBits));
+                k = e;
+            }
+        }
+    }
     return pl;
 }
 

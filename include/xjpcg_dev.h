@@ -103,8 +103,11 @@ XJPCG_Status_t xjpcg_op_set_vectors(xjpcg_op_t* op, int bits, int* h_bits);
  * calls): xjpcg_panel_dims gives the rows per block, the row blocks and the
  * column panels for n rows; xjpcg_panel_plan fills seg[nb*P+1], off[nnz],
  * lrow[nnz] and perm[nnz] (layout entry q holds CSR entry perm[q]) from a host
- * CSR.  *h_ordered = 0 when some row's columns are not in ascending panel
- * order.  No counterpart in the reference; for inspection and tests. */
+ * CSR.  lrow[q] & 0xFFF is the entry's row offset in its block; lrow[q] >> 12
+ * is, on the first entry of a row's run inside each 2048-entry chunk of a
+ * segment, min(run length, 15), else 0.  *h_ordered = 0 when some row's
+ * columns are not in ascending panel order.  No counterpart in the
+ * reference; for inspection and tests. */
 XJPCG_Status_t xjpcg_panel_dims(int32_t n, int32_t* block_rows, int32_t* nb, int32_t* npanels);
 XJPCG_Status_t xjpcg_panel_plan(int32_t n, const int32_t* h_rowptr, const int32_t* h_col, int32_t* seg,
                                 uint16_t* off, uint16_t* lrow, int64_t* perm, int* h_ordered);

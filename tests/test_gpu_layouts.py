@@ -58,6 +58,25 @@ def test_panel_spmv_bitexact_all_rows(cuda):
     assert np.array_equal(yt[short], ref[short])
 
 
+def test_panel_spmv_every_run_length(cuda):
+    """Rows with 0..99 entries in one panel: every run length the row words encode (1..14), the scanning
+    path (15 and more, ends at each offset of its 8-entry steps) and runs cut by a chunk's end, bit-exact."""
+    from hpc_amd import device
+    n = 140_000
+    rng = np.random.default_rng(21)
+    deg = np.arange(n) % 100
+    rows = np.repeat(np.arange(n), deg)
+    cols = rng.integers(0, 65536, rows.size)
+    vals = rng.standard_normal(rows.size)
+    o = np.lexsort((cols, rows))
+    rp, col, val, _ = oracle.coo_to_csr(n, rows[o], cols[o], vals[o])
+    op = device.CsrOperator(_t(cuda, rp), _t(cuda, col), _t(cuda, val), rowptr_host=rp)
+    assert op.set_layout(2) == 2
+    xv = workloads.uniform(5, 7, n, -1, 1)
+    y = op.spmv(_t(cuda, xv)).cpu().numpy()
+    assert np.array_equal(y, oracle.spmv(rp, col, val, xv, threads=8))
+
+
 def test_panel_solve_matches_oracle(cuda):
     from hpc_amd import device
     A = workloads.graph_laplacian(300_000, seed=7)

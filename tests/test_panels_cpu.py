@@ -49,7 +49,8 @@ def test_panel_plan_invariants_and_emulated_sums():
     # segment of every layout entry -> (block, panel)
     q_seg = np.repeat(np.arange(seg.size - 1), np.diff(seg))
     b, p = q_seg // P, q_seg % P
-    rows = b * br + pl["lrow"].astype(np.int64)
+    word = pl["lrow"].astype(np.int64)
+    rows = b * br + (word & 0xFFF)
     cols = p * 65536 + pl["off"].astype(np.int64)
     assert np.array_equal(rows, csr_row[perm])
     assert np.array_equal(cols, col[perm])
@@ -58,6 +59,16 @@ def test_panel_plan_invariants_and_emulated_sums():
     assert np.all(rows[1:][same] >= rows[:-1][same])
     order = np.lexsort((np.arange(nnz), rows))
     assert np.all(np.diff(perm[order])[rows[order][1:] == rows[order][:-1]] > 0)
+    # run codes: inside each staged chunk of 2048 entries of a segment, the first entry of a row's run
+    # carries min(run length, 15) above the 12-bit row offset, every other entry 0
+    code = word >> 12
+    pos = np.arange(nnz) - seg[q_seg]
+    start = (pos % 2048 == 0) | np.concatenate(([True], rows[1:] != rows[:-1])) | np.concatenate(([True], same == 0))
+    assert np.all(code[~start] == 0) and np.all(code[start] > 0)
+    run_id = np.cumsum(start) - 1
+    run_len = np.bincount(run_id)
+    assert np.array_equal(code[start], np.minimum(run_len, 15))
+    assert run_len.max() > 15                    # the long rows take the scanning path
     # emulate the kernel: per row, products added in layout order starting from 0
     x = workloads.uniform(3, 1, n, -1, 1)
     y = np.zeros(n)
