@@ -515,15 +515,18 @@ __global__ __launch_bounds__(kBlock) void spmv_tiles(int ntiles, int nnz, const 
 // row-major with ascending columns.  A chunk of up to kPanelChunk entries is
 // loaded (value, uint16 column offset, uint16 row word: row offset and, on a
 // run's first entry, the run's length), the products are staged in LDS, and
-// the lane holding the first entry of each row's run in the chunk adds that
-// run to the row's sum left to right.  With the length known, its loads of
-// the run's products go out together: walking the run to find its end cost
-// one LDS round trip per entry, and every chunk waited at its barrier for the
-// longest run (4M-row random graph: 746 us against 533 us with no gathers at
-// all, 220 us without the run sums).  A row's runs in
-// later chunks and panels hold larger columns and are added after it, so
-// every row is summed left to right from dsplit[i]*x[i] (or 0), exactly as
-// the tile kernel sums a short row.
+// each row's run in the chunk is added to the row's sum in one of two bins.
+// A run of fewer than kPanelRunScan entries is added left to right by the
+// lane holding its first entry; with the length known, its loads of the run's
+// products go out together (walking the run to find its end cost one LDS
+// round trip per entry, and every chunk waited at its barrier for the longest
+// run: 4M-row random graph, 746 us against 533 us with no gathers at all and
+// 220 us without the run sums).  A longer run is summed by the wavefront
+// holding its first entry in a fixed order (64 entries per step, an xor
+// butterfly) and its total added to the row's sum.  A row's runs in later
+// chunks and panels hold larger columns and are added after it, so a row
+// whose runs are all short is summed left to right from dsplit[i]*x[i] (or
+// 0), exactly as the tile kernel sums a short row.
 constexpr int kPanelCols = kPanelColsHost;
 constexpr int kPanelBlock = kPanelBlockHost;
 constexpr int kPanelChunk = kPanelChunkHost;
@@ -564,6 +567,10 @@ __device__ __forceinline__ void panel_rows(const PanelDev& A, int n, int b, cons
                 o[j] = off[e];
                 w[j] = lrow[e];
             }
+            // every stream load of the chunk is issued before anything waits
+            // for one (left to itself the scheduler held a value load behind
+            // the first column load's return: one more HBM latency per chunk)
+            __builtin_amdgcn_sched_barrier(0);
             double pr[J];
 #pragma unroll
             for (int j = 0; j < J; ++j) {
@@ -573,53 +580,58 @@ __device__ __forceinline__ void panel_rows(const PanelDev& A, int n, int b, cons
                 lr[k] = k < cn ? w[j] : (uint16_t)0xFFFF;
             }
             __syncthreads();
-            // the lane holding a run's first entry adds the run: its own
-            // product, then the run's other products from LDS, all loaded
-            // before the first add when the run's length is in the word
+            // the lane holding a run's first entry adds a short run (its
+            // length in the word): its own product, then the run's other
+            // products from LDS, all loaded before the first add
+            bool wide[J];
+            int rw[J];
 #pragma unroll
             for (int j = 0; j < J; ++j) {
                 const int k = tid + j * kPanelBlock;
                 const int len = w[j] >> kPanelRowBits;
-                if (k < cn && len != 0) {
-                    const int r = w[j] & ((1 << kPanelRowBits) - 1);
+                rw[j] = w[j] & ((1 << kPanelRowBits) - 1);
+                wide[j] = k < cn && len >= kPanelRunScan;
+                if (k < cn && len != 0 && len < kPanelRunScan) {
+                    const int r = rw[j];
                     double a = acc[r];
                     a += pr[j];
                     int q = k + 1;
-                    if (len < kPanelRunScan) {
-                        const int end = k + len;
-                        for (; q + 3 < end; q += 4) {
-                            const double t0 = prod[q], t1 = prod[q + 1], t2 = prod[q + 2], t3 = prod[q + 3];
-                            a += t0;
-                            a += t1;
-                            a += t2;
-                            a += t3;
-                        }
-                        for (; q < end; ++q) a += prod[q];
-                    } else {
-                        // a long run (a high-degree row): its end is found
-                        // 8 entries at a time, each step's row words and
-                        // products loaded together (continuation words hold
-                        // the bare row offset; slots past the chunk hold
-                        // 0xFFFF)
-                        bool more = true;
-                        while (more) {
-                            uint16_t ww[8];
-                            double tt[8];
-#pragma unroll
-                            for (int u = 0; u < 8; ++u) {
-                                const int qq = min(q + u, kPanelChunk - 1);
-                                ww[u] = lr[qq];
-                                tt[u] = prod[qq];
-                            }
-#pragma unroll
-                            for (int u = 0; u < 8; ++u) {
-                                more = more && q + u < cn && ww[u] == r;
-                                if (more) a += tt[u];
-                            }
-                            q += 8;
-                        }
+                    const int end = k + len;
+                    for (; q + 3 < end; q += 4) {
+                        const double t0 = prod[q], t1 = prod[q + 1], t2 = prod[q + 2], t3 = prod[q + 3];
+                        a += t0;
+                        a += t1;
+                        a += t2;
+                        a += t3;
                     }
+                    for (; q < end; ++q) a += prod[q];
                     acc[r] = a;
+                }
+            }
+            // a long run (kPanelRunScan or more entries, a high-degree row):
+            // the wavefront holding its first entry sums it, 64 entries per
+            // step (lane l adds entries start+l, start+l+64, ... in order,
+            // then the xor butterfly); the run ends at the first entry whose
+            // row word differs (continuation words hold the bare row offset,
+            // slots past the chunk 0xFFFF)
+            const int lane = tid & 63;
+#pragma unroll
+            for (int j = 0; j < J; ++j) {
+                uint64_t m = __ballot(wide[j]);
+                while (m) {
+                    const int l = __builtin_ctzll(m);
+                    m &= m - 1;
+                    const int k0r = (tid - lane + l) + j * kPanelBlock;  // the run's first entry
+                    const int r = __shfl(rw[j], l, 64);
+                    double part = 0.0;
+                    for (int q = k0r + lane;; q += 64) {
+                        const int qq = min(q, kPanelChunk - 1);
+                        const bool in = q < cn && (q == k0r || lr[qq] == r);
+                        if (in) part += prod[qq];
+                        if (__ballot(!in) != 0) break;
+                    }
+                    part = wave_sum(part);
+                    if (lane == l) acc[r] = acc[r] + part;
                 }
             }
             __syncthreads();

@@ -14,6 +14,7 @@ constexpr int kBlock = 256;                // threads per workgroup (4 wavefront
 constexpr int kWaves = kBlock / 64;
 constexpr int kTileNnz = kTileNnzHost;     // max stored entries per short-row tile (16 KB of fp64 products in LDS)
 constexpr int kTileRows = kTileRowsHost;   // max rows per tile: one lane reduces one row
+constexpr int kWaveRowMin = kWaveRowMinHost;  // longer rows of a short-row tile: summed by their wavefront
 static_assert(kTileRows == kBlock, "one lane per tile row");
 static_assert(kTileNnz % kBlock == 0, "whole loads per lane");
 
@@ -46,6 +47,34 @@ __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
+}
+
+// Wavefront bin of a row-binned SpMV.  Each lane of the wavefront may own a
+// row whose products sit in lds[a, b); the lanes with wide == true (rows longer
+// than kWaveRowMin) are served one after another in lane order by the whole
+// wavefront: lane l adds entries a+l, a+l+64, ... left to right, the 64
+// partials are folded by the xor butterfly, and the owner adds the total to
+// its s.  The order depends only on the row's length, so the sum is
+// deterministic.  Called by all 64 lanes of the wavefront together.
+__device__ __forceinline__ void wave_rows_sum(bool wide, int a, int b, const double* lds, double& s) {
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
+    uint64_t m = __ballot(wide);
+    while (m) {
+        const int l = __builtin_ctzll(m);
+        m &= m - 1;
+        const int ra = __shfl(a, l, 64), rb = __shfl(b, l, 64);
+        double p = 0.0;
+        int j = ra + lane;
+        for (; j + 64 < rb; j += 128) {
+            const double q0 = lds[j], q1 = lds[j + 64];
+            p += q0;
+            p += q1;
+        }
+        if (j < rb) p += lds[j];
+        p = wave_sum(p);
+        if (lane == l) s += p;
+    }
 }
 
 // Sum over a workgroup of NW wavefronts; every thread returns the identical
@@ -171,9 +200,11 @@ __device__ __forceinline__ T ld_stream(const T* p) {
 // rows with at most kTileRows rows and kTilePlanNnz stored entries, or one row
 // holding more than kTilePlanNnz entries.  For a short-row tile, lanes stream
 // the tile's values and column indices contiguously with 16-byte
-// non-temporal loads, gather x, and park the products in LDS; then lane t
-// sums row t's products left to right.  A long row is summed by the whole
-// workgroup with a strided loop and a fixed-order workgroup reduction.
+// non-temporal loads, gather x, and park the products in LDS; then the rows
+// are summed in two bins: lane t sums row t's products left to right when the
+// row has at most kWaveRowMin entries, and the lane's wavefront sums a longer
+// row together (wave_rows_sum).  A row longer than a tile is summed by the
+// whole workgroup with a strided loop and a fixed-order workgroup reduction.
 //
 // VT is the stored value type (double, or float for the mixed-precision
 // inner operator).  With dsplit != nullptr the matrix holds no diagonal and
@@ -269,7 +300,8 @@ __device__ __forceinline__ void row_tile_load(RowTile<VT, CT>& T, int t, const i
 }
 
 // Compute of a loaded row tile: gathers (all of a lane's in flight
-// together), products in LDS, lane-per-row left-to-right sums; a long row is
+// together), products in LDS, binned row sums (lane per row up to
+// kWaveRowMin entries, wavefront per row above); a row longer than a tile is
 // summed by the whole workgroup with a strided loop and a fixed-order
 // workgroup reduction.  On return lanes with *row >= 0 hold y[*row] in *y;
 // ends with a __syncthreads so the workgroup's next tile can reuse lds / red.
@@ -316,11 +348,13 @@ __device__ __forceinline__ void row_tile_compute(const RowTile<VT, CT>& T, const
         }
     }
     __syncthreads();
-    if (tid < r1 - r0) {
-        // a lane's row bounds load here, not with the tile: holding them across
-        // the gathers costs more (measured) than the load
-        const int a = rowptr[r0 + tid] - k0, b = rowptr[r0 + tid + 1] - k0;
-        double s = dsplit ? dsplit[r0 + tid] * x[r0 + tid] : 0.0;
+    const bool mine = tid < r1 - r0;
+    // a lane's row bounds load here, not with the tile: holding them across
+    // the gathers costs more (measured) than the load
+    const int a = mine ? rowptr[r0 + tid] - k0 : 0, b = mine ? rowptr[r0 + tid + 1] - k0 : 0;
+    double s = (mine && dsplit) ? dsplit[r0 + tid] * x[r0 + tid] : 0.0;
+    const bool wide = b - a > kWaveRowMin;
+    if (mine && !wide) {
         // left-to-right sum; the LDS reads of a group of four issue together
         int j = a;
         for (; j + 4 <= b; j += 4) {
@@ -331,11 +365,10 @@ __device__ __forceinline__ void row_tile_compute(const RowTile<VT, CT>& T, const
             s += q3;
         }
         for (; j < b; ++j) s += lds[j];
-        *y = s;
-        *row = r0 + tid;
-    } else {
-        *row = -1;
     }
+    wave_rows_sum(wide, a, b, lds, s);
+    *y = s;
+    *row = mine ? r0 + tid : -1;
     __syncthreads();
 }
 

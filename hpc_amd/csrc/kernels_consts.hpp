@@ -10,6 +10,12 @@ constexpr int kTileRowsHost = 256;   // rows per tile (one lane per row)
 // fp32 values), so a tile holds at most kTileNnzHost - 4 entries.  A row with
 // more entries is a tile of its own, summed by the whole workgroup.
 constexpr int kTilePlanNnz = kTileNnzHost - 4;
+// Row bins of the tile SpMV: inside a short-row tile, a row with at most
+// kWaveRowMinHost entries is summed by its own lane left to right; a longer
+// one by its wavefront (lane l adds entries l, l+64, ... in order, then a
+// fixed xor butterfly).  Rows above kTilePlanNnz are tiles of their own,
+// summed by the whole workgroup.
+constexpr int kWaveRowMinHost = 64;
 }  // namespace xj
 
 namespace xj {
@@ -24,8 +30,8 @@ constexpr int kPanelBlockHost = 1024;
 constexpr int kPanelChunkHost = 2048;
 // An entry's uint16 row word: the row offset in its block in the low
 // kPanelRowBits bits; above them, for the first entry of a row's run inside a
-// staged chunk, the run's length (kPanelRunScan: that many or more, the
-// kernel scans for the end), 0 for the run's other entries.
+// staged chunk, the run's length (kPanelRunScan: that many or more, summed by
+// a wavefront that scans for the run's end), 0 for the run's other entries.
 constexpr int kPanelRowBits = 12;
 constexpr int kPanelRunScan = 15;
 static_assert(kPanelRowsMaxHost <= (1 << kPanelRowBits) && kPanelRunScan < (1 << (16 - kPanelRowBits)),

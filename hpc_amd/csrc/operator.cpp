@@ -157,8 +157,9 @@ PanelPlan plan_panels(int32_t n, const int32_t* rp, const int32_t* col, int bloc
     }
     // run codes: the kernels stage a segment kPanelChunkHost entries at a
     // time; the first entry of each row's run inside such a chunk carries the
-    // run's length (capped at kPanelRunScan, "scan for the end") in the bits
-    // above the row offset, the other entries carry 0
+    // run's length (capped at kPanelRunScan: a wavefront sums the run and
+    // scans for its end) in the bits above the row offset, the other entries
+    // carry 0
     for (int64_t q = 0; q < nseg; ++q) {
         const int64_t s1 = pl.seg[(size_t)q + 1];
         for (int64_t c0 = pl.seg[(size_t)q]; c0 < s1; c0 += kPanelChunkHost) {

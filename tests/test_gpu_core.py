@@ -9,6 +9,8 @@ import pytest
 from hpc_amd import workloads
 from oracle import oracle
 
+import sum_order
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLDEN = os.path.join(HERE, "golden")
 
@@ -55,6 +57,9 @@ def test_spmv_golden_cases(cuda, case):
         assert np.array_equal(y, y_orc)
     else:
         assert np.allclose(y, y_orc, rtol=1e-13, atol=1e-12)
+        # row bins: lane, wavefront and workgroup sums, each in its fixed order
+        assert op.layout == 1
+        assert np.array_equal(y, sum_order.tile_spmv(rp, col, val, z[f"x{case}"]))
 
 
 @pytest.mark.parametrize("kind,grid", [(0, (64, 48, 1)), (1, (20, 17, 9)), (2, (24, 24, 24))])
@@ -73,9 +78,10 @@ def test_spmv_graph_rows(cuda):
     x = workloads.uniform(5, 9, A.n, -1, 1)
     y = to_dev(cuda, A).spmv(dvec(cuda, x)).cpu().numpy()
     ref = oracle.spmv(A.rowptr, A.col, A.val, x)
-    short = np.diff(A.rowptr) <= 2048
-    assert np.array_equal(y[short], ref[short])
-    assert np.allclose(y, ref, rtol=1e-13, atol=1e-12)
+    short = np.diff(A.rowptr) <= sum_order.WAVE_MIN
+    assert np.array_equal(y[short], ref[short])             # lane per row: left to right
+    assert np.array_equal(y, sum_order.tile_spmv(A.rowptr, A.col, A.val, x))   # wavefront per longer row
+    assert np.all(np.abs(y - ref) <= 2 * sum_order.sum_bound(A.rowptr, A.col, A.val, x))
 
 
 def test_spmv_closed_form_row_sums(cuda):

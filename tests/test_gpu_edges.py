@@ -11,6 +11,8 @@ import pytest
 from hpc_amd import workloads
 from oracle import oracle
 
+import sum_order
+
 pytestmark = pytest.mark.gpu
 
 
@@ -74,7 +76,8 @@ def test_spmv_edges_every_layout(cuda, name, f32):
                             rowptr_host=rp)
     xv = workloads.uniform(3, 1, n, -1, 1)
     ref = oracle.spmv(rp, col, val, xv)
-    short = np.diff(rp) <= 2044
+    short = np.diff(rp) <= sum_order.WAVE_MIN
+    bound = 2 * sum_order.sum_bound(rp, col, val, xv)
     ran = []
     for lay, lname in LAYOUTS.items():
         try:
@@ -84,10 +87,13 @@ def test_spmv_edges_every_layout(cuda, name, f32):
         y = op.spmv(_t(cuda, xv)).cpu().numpy()
         ran.append(lname)
         if lname == "tiles":
-            assert np.array_equal(y[short], ref[short]), lname
-            assert np.allclose(y, ref, rtol=1e-12, atol=1e-11), lname
+            assert np.array_equal(y[short], ref[short]), lname   # lane per row: left to right
+            assert np.array_equal(y, sum_order.tile_spmv(rp, col, val, xv)), lname
+        elif lname == "panels":
+            assert np.array_equal(y, sum_order.panel_spmv(device.panel_plan(rp, col), n, val, xv)), lname
         else:
             assert np.array_equal(y, ref), lname        # left to right in every row
+        assert np.all(np.abs(y - ref) <= bound), lname
     _say(f"{name} {'fp32' if f32 else 'fp64'}: layouts {ran}")
     assert "tiles" in ran and "panels" in ran
 

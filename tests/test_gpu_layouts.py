@@ -10,6 +10,8 @@ import pytest
 from hpc_amd import workloads
 from oracle import oracle
 
+import sum_order
+
 pytestmark = pytest.mark.gpu
 
 
@@ -45,22 +47,30 @@ def test_panel_spmv_bitexact_all_rows(cuda):
     xv = workloads.uniform(5, 3, n, -1, 1)
     x = _t(cuda, xv)
     ref = oracle.spmv(rp, col, val, xv, threads=8)
+    bound = 2 * sum_order.sum_bound(rp, col, val, xv)
     y = op.spmv(x).cpu().numpy()
-    assert np.array_equal(y, ref)               # left-to-right in every row, long rows too
+    # runs of a row inside a staged chunk: short ones left to right, long ones (the 3000-entry rows) by a
+    # wavefront in its fixed order
+    assert np.array_equal(y, sum_order.panel_spmv(device.panel_plan(rp, col), n, val, xv))
+    assert np.array_equal(y[np.diff(rp) < sum_order.RUN_SCAN], ref[np.diff(rp) < sum_order.RUN_SCAN])
+    assert np.all(np.abs(y - ref) <= bound)
     # residual on the same layout
     bv = workloads.uniform(5, 4, n, -1, 1)
     r, rr = op.residual(_t(cuda, bv), x)
-    assert np.array_equal(r.cpu().numpy(), bv - ref)
-    # tiles agree on every row short enough for a tile
+    assert np.array_equal(r.cpu().numpy(), bv - y)
+    # tiles: left to right on rows up to the wavefront bin, each bin in its own order
     assert op.set_layout(1) == 1
     yt = op.spmv(x).cpu().numpy()
-    short = np.diff(rp) <= 2044
+    short = np.diff(rp) <= sum_order.WAVE_MIN
     assert np.array_equal(yt[short], ref[short])
+    assert np.array_equal(yt, sum_order.tile_spmv(rp, col, val, xv))
+    assert np.all(np.abs(yt - ref) <= bound)
 
 
 def test_panel_spmv_every_run_length(cuda):
-    """Rows with 0..99 entries in one panel: every run length the row words encode (1..14), the scanning
-    path (15 and more, ends at each offset of its 8-entry steps) and runs cut by a chunk's end, bit-exact."""
+    """Rows with 0..99 entries in one panel: every run length the row words encode (1..14, left to right),
+    the wavefront path (15 and more: ends at every lane of a 64-entry step) and runs cut by a chunk's end,
+    bit-exact against the kernel's order."""
     from hpc_amd import device
     n = 140_000
     rng = np.random.default_rng(21)
@@ -74,7 +84,10 @@ def test_panel_spmv_every_run_length(cuda):
     assert op.set_layout(2) == 2
     xv = workloads.uniform(5, 7, n, -1, 1)
     y = op.spmv(_t(cuda, xv)).cpu().numpy()
-    assert np.array_equal(y, oracle.spmv(rp, col, val, xv, threads=8))
+    ref = oracle.spmv(rp, col, val, xv, threads=8)
+    assert np.array_equal(y, sum_order.panel_spmv(device.panel_plan(rp, col), n, val, xv))
+    assert np.array_equal(y[deg < sum_order.RUN_SCAN], ref[deg < sum_order.RUN_SCAN])
+    assert np.all(np.abs(y - ref) <= 2 * sum_order.sum_bound(rp, col, val, xv))
 
 
 def test_panel_solve_matches_oracle(cuda):
@@ -134,7 +147,12 @@ def test_panel_split_fp32_and_mixed(cuda):
     yt = inner.spmv(x).clone()
     assert inner.set_layout(2) == 2
     yp = inner.spmv(x)
-    short = torch.from_numpy(np.diff(srp) <= 2044).to(cuda)
+    # each against its kernel's order (the graph's rows reach the wavefront bins of both layouts)
+    xh = x.cpu().numpy()
+    assert np.array_equal(yt.cpu().numpy(), sum_order.tile_spmv(srp, scol, sv32, xh, dsplit=sdiag))
+    assert np.array_equal(yp.cpu().numpy(),
+                          sum_order.panel_spmv(device.panel_plan(srp, scol), A.n, sv32, xh, dsplit=sdiag))
+    short = torch.from_numpy(np.diff(srp) < sum_order.RUN_SCAN).to(cuda)
     assert bool((yt[short] == yp[short]).all())
     assert outer.set_layout(2) == 2
     xm, info = device.solve_mixed(outer, inner, _t(cuda, A.diag), _t(cuda, b), 1e-10)

@@ -11,6 +11,8 @@ from hpc_amd import device, workloads
 from hpc_amd._lib import XjpcgError
 from oracle import oracle
 
+import sum_order
+
 
 def scattered_csr(n, seed, max_deg=40):
     rng = np.random.default_rng(seed)
@@ -68,12 +70,22 @@ def test_panel_plan_invariants_and_emulated_sums():
     run_id = np.cumsum(start) - 1
     run_len = np.bincount(run_id)
     assert np.array_equal(code[start], np.minimum(run_len, 15))
-    assert run_len.max() > 15                    # the long rows take the scanning path
-    # emulate the kernel: per row, products added in layout order starting from 0
+    assert run_len.max() > 15                    # the long rows take the wavefront path
+    # emulate the kernel: per row, products added in layout order starting from 0 -- left to right, the
+    # oracle's order, wherever no run takes the wavefront path
     x = workloads.uniform(3, 1, n, -1, 1)
     y = np.zeros(n)
     np.add.at(y, rows, val[perm] * x[cols])
-    assert np.array_equal(y, oracle.spmv(rp, col, val, x))
+    ref = oracle.spmv(rp, col, val, x)
+    assert np.array_equal(y, ref)
+    # the kernel's binned order: equal to it for rows whose runs are all short, within the summation bound
+    # for the others
+    yb = sum_order.panel_spmv(pl, n, val, x)
+    wide_rows = np.unique(rows[start][code[start] >= 15])
+    short = np.ones(n, bool)
+    short[wide_rows] = False
+    assert wide_rows.size > 0 and np.array_equal(yb[short], ref[short])
+    assert np.all(np.abs(yb - ref) <= 2 * sum_order.sum_bound(rp, col, val, x))
 
 
 def test_panel_plan_flags_unordered_rows():
