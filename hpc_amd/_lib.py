@@ -79,6 +79,7 @@ _SIGNATURES = {
     "xjpcg_batch_set_config": (c_int, [c_void_p, c_int32, c_int32, c_int32]),
     "xjpcg_batch_set_layout": (c_int, [c_void_p, c_int32, POINTER(c_int32), c_void_p]),
     "xjpcg_batch_set_vectors": (c_int, [c_void_p, c_int32, POINTER(c_int32)]),
+    "xjpcg_batch_set_path": (c_int, [c_void_p, c_int32, c_int32, POINTER(c_int32), POINTER(c_int32)]),
     "xjpcg_batch_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_double, c_void_p,
                                   c_void_p, POINTER(c_int64), c_void_p]),
     "xjpcg_batch_solve_mixed": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_double,

@@ -144,6 +144,18 @@ class Batch:
         _check(lib().xjpcg_batch_set_vectors(self._h, 0, ctypes.byref(act)))
         return act.value
 
+    def set_path(self, path: int, grid: int = 0):
+        """Iteration loop of solve(): 0 auto / 2 graph-replayed launches, 1 one persistent cooperative launch of
+        `grid` workgroups (0: the default size); both give the same bits."""
+        _check(lib().xjpcg_batch_set_path(self._h, int(path), int(grid), None, None))
+
+    @property
+    def last_path(self) -> tuple[int, int]:
+        """(loop the last solve used, its persistent grid)"""
+        p, g = ctypes.c_int32(), ctypes.c_int32()
+        _check(lib().xjpcg_batch_set_path(self._h, -1, 0, ctypes.byref(p), ctypes.byref(g)))
+        return p.value, g.value
+
     def configure(self, tile_rows: int = 0, block: int = 0, tune: bool = False):
         _check(lib().xjpcg_batch_set_config(self._h, int(tile_rows), int(block), 1 if tune else 0))
 

@@ -38,24 +38,39 @@ __device__ __forceinline__ double group_sum(double v, double* red) {
     return s;
 }
 
-// Sum of part[t*R + j] over the tiles t of system s, fixed order.  Each lane
-// issues four loads before adding them in order, so a large system's partials
-// cost a quarter of the dependent round trips.
+// Sum of part[t*R + j] over the tiles t of system s in a fixed order that
+// does not depend on the block size: kMaxBlock / R virtual groups, virtual
+// group v adding the tiles t0 + v, t0 + v + kMaxBlock/R, ... in order, then
+// the groups' sums added in order v = 0, 1, ...  A thread of a smaller
+// workgroup carries several virtual groups, so the system kernels (512
+// threads) and the persistent loop (the tile kernels' block size) produce
+// the same bits.  Each virtual group issues four loads before adding them in
+// order, so a large system's partials cost a quarter of the dependent round
+// trips.  Lanes with g == 0 return the total.
 template <int R>
 __device__ __forceinline__ double system_sum(const double* part, int t0, int t1, double* red) {
+    constexpr int GV = kMaxBlock / R;
     const int j = threadIdx.x % R, g = threadIdx.x / R, G = blockDim.x / R;
-    double s = 0.0;
-    int t = t0 + g;
-    for (; t + 3 * G < t1; t += 4 * G) {
-        const double a0 = part[(size_t)t * R + j], a1 = part[(size_t)(t + G) * R + j];
-        const double a2 = part[(size_t)(t + 2 * G) * R + j], a3 = part[(size_t)(t + 3 * G) * R + j];
-        s += a0;
-        s += a1;
-        s += a2;
-        s += a3;
+    for (int v = g; v < GV; v += G) {
+        double s = 0.0;
+        int t = t0 + v;
+        for (; t + 3 * GV < t1; t += 4 * GV) {
+            const double a0 = part[(size_t)t * R + j], a1 = part[(size_t)(t + GV) * R + j];
+            const double a2 = part[(size_t)(t + 2 * GV) * R + j], a3 = part[(size_t)(t + 3 * GV) * R + j];
+            s += a0;
+            s += a1;
+            s += a2;
+            s += a3;
+        }
+        for (; t < t1; t += GV) s += part[(size_t)t * R + j];
+        red[v * R + j] = s;
     }
-    for (; t < t1; t += G) s += part[(size_t)t * R + j];
-    return group_sum<R>(s, red);
+    __syncthreads();
+    double s = 0.0;
+    if (g == 0)
+        for (int v = 0; v < GV; ++v) s += red[v * R + j];
+    __syncthreads();
+    return s;
 }
 
 // Work vectors r, p, Ap are stored as WT: double, or float for the inner
@@ -251,11 +266,10 @@ __device__ __forceinline__ void brow_dia(const BatchParams& P, const BDiaSys& Y,
     }
 }
 
+// Ap = A p on tile t and its p.Ap partials partA[t][R]; red: kMaxBlock * Q doubles of LDS
 template <typename VT, int R, bool DIA, typename WT>
-__global__ __launch_bounds__(kMaxBlock) void bspmm_dot(BatchParams P) {
+__device__ __forceinline__ void bspmm_dot_tile(const BatchParams& P, int t, double* red) {
     constexpr int Q = SpmmLanes<R>::Q, L = SpmmLanes<R>::L;
-    __shared__ double red[kMaxBlock * Q];
-    const int t = (int)blockIdx.x;
     const int s = P.tile_sys[t];
     BDiaSys Y;
     if (DIA) Y = P.dsys[s];
@@ -294,6 +308,13 @@ __global__ __launch_bounds__(kMaxBlock) void bspmm_dot(BatchParams P) {
         for (int gg = 0; gg < G; ++gg) sum += red[gg * R + j];
         P.partA[(size_t)t * R + j] = sum;
     }
+    __syncthreads();  // red is reused by the workgroup's next tile
+}
+
+template <typename VT, int R, bool DIA, typename WT>
+__global__ __launch_bounds__(kMaxBlock) void bspmm_dot(BatchParams P) {
+    __shared__ double red[kMaxBlock * SpmmLanes<R>::Q];
+    bspmm_dot_tile<VT, R, DIA, WT>(P, (int)blockIdx.x, red);
 }
 
 // plain batched SpMM y = A x (or y = b - A x) for the autotuner and the refinement residual
@@ -322,10 +343,9 @@ __global__ __launch_bounds__(kMaxBlock) void bspmm(BatchParams P, const double* 
     }
 }
 
+// alpha per pair of system s (red: kMaxBlock doubles of LDS)
 template <int R>
-__global__ __launch_bounds__(kMaxBlock) void bsys_alpha(BatchParams P) {
-    __shared__ double red[kMaxBlock];
-    const int s = blockIdx.x;
+__device__ __forceinline__ void bsys_alpha_sys(const BatchParams& P, int s, double* red) {
     if (P.sys_active[s]) {
         const int j = threadIdx.x % R, g = threadIdx.x / R;
         const double pap = system_sum<R>(P.partA, P.sys_tile[s], P.sys_tile[s + 1], red);
@@ -340,10 +360,15 @@ __global__ __launch_bounds__(kMaxBlock) void bsys_alpha(BatchParams P) {
     if (s == 0 && threadIdx.x == 0) *P.any_active = 0;
 }
 
-template <int R, typename WT>
-__global__ __launch_bounds__(kMaxBlock) void bupdate_r(BatchParams P) {
+template <int R>
+__global__ __launch_bounds__(kMaxBlock) void bsys_alpha(BatchParams P) {
     __shared__ double red[kMaxBlock];
-    const int t = (int)blockIdx.x;
+    bsys_alpha_sys<R>(P, (int)blockIdx.x, red);
+}
+
+// r -= alpha Ap on tile t and its r.r, r.z partials (red: kMaxBlock doubles of LDS)
+template <int R, typename WT>
+__device__ __forceinline__ void bupdate_r_tile(const BatchParams& P, int t, double* red) {
     const int s = P.tile_sys[t];
     if (!P.sys_active[s]) return;
     const int j = threadIdx.x % R, g = threadIdx.x / R, G = blockDim.x / R;
@@ -386,11 +411,18 @@ __global__ __launch_bounds__(kMaxBlock) void bupdate_r(BatchParams P) {
     }
 }
 
-template <int R>
-__global__ __launch_bounds__(kMaxBlock) void bsys_beta(BatchParams P) {
+template <int R, typename WT>
+__global__ __launch_bounds__(kMaxBlock) void bupdate_r(BatchParams P) {
     __shared__ double red[kMaxBlock];
-    __shared__ int any_active, any_upd;
-    const int s = blockIdx.x;
+    bupdate_r_tile<R, WT>(P, (int)blockIdx.x, red);
+}
+
+// beta, the stop rule and the masks per pair of system s (red: kMaxBlock
+// doubles of LDS; flags: two ints of LDS)
+template <int R>
+__device__ __forceinline__ void bsys_beta_sys(const BatchParams& P, int s, double* red, int* flags) {
+    int& any_active = flags[0];
+    int& any_upd = flags[1];
     const int j = threadIdx.x % R, g = threadIdx.x / R;
     if (!P.sys_active[s]) {
         if (g == 0) P.upd[s * R + j] = 0;
@@ -435,11 +467,19 @@ __global__ __launch_bounds__(kMaxBlock) void bsys_beta(BatchParams P) {
         P.sys_upd[s] = any_upd;
         if (any_active) *P.any_active = 1;
     }
+    __syncthreads();  // flags are reused by the workgroup's next system
 }
 
+template <int R>
+__global__ __launch_bounds__(kMaxBlock) void bsys_beta(BatchParams P) {
+    __shared__ double red[kMaxBlock];
+    __shared__ int flags[2];
+    bsys_beta_sys<R>(P, (int)blockIdx.x, red, flags);
+}
+
+// x += alpha p; p = beta p + J r on tile t for the pairs that ran this iteration
 template <int R, typename WT>
-__global__ __launch_bounds__(kMaxBlock) void bupdate_xp(BatchParams P) {
-    const int t = (int)blockIdx.x;
+__device__ __forceinline__ void bupdate_xp_tile(const BatchParams& P, int t) {
     const int s = P.tile_sys[t];
     if (!P.sys_upd[s]) return;
     const int j = threadIdx.x % R, g = threadIdx.x / R, G = blockDim.x / R;
@@ -472,6 +512,11 @@ __global__ __launch_bounds__(kMaxBlock) void bupdate_xp(BatchParams P) {
             }
         }
     }
+}
+
+template <int R, typename WT>
+__global__ __launch_bounds__(kMaxBlock) void bupdate_xp(BatchParams P) {
+    bupdate_xp_tile<R, WT>(P, (int)blockIdx.x);
 }
 
 __global__ void bfinish(int nq, const double* __restrict__ dot, const double* __restrict__ rr,
@@ -522,6 +567,48 @@ __global__ __launch_bounds__(kMaxBlock) void bmask(BatchParams P, const int* __r
 __global__ void baxpy(size_t m, const double* __restrict__ d, double* __restrict__ x) {
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += (size_t)gridDim.x * blockDim.x)
         x[e] += d[e];
+}
+
+// --------------------------------------------------------------------------
+// One persistent launch for the whole iteration loop of a batch.
+//
+// The grid (all workgroups co-resident, cooperative launch) runs the same
+// five steps per iteration as the launch-per-step path, with a grid barrier
+// after each: tiles w, w+G, ... of the SpMM, systems w, w+G, ... of the alpha
+// step, tiles of the r update, systems of the beta / stop / mask step, tiles
+// of the x / p update.  Every tile and system step is the launch-per-step
+// path's own code with the same reduction orders (system sums do not depend
+// on the block size), so both paths produce the same bits; converged
+// systems drop out through the same per-system and per-pair masks.  After
+// each iteration every workgroup reads the any-active flag the beta step
+// left, so all of them leave the loop together.  A barrier wait longer than
+// its timeout sets the abort flag and every workgroup leaves.
+template <typename VT, int R, bool DIA, typename WT>
+__global__ __launch_bounds__(kMaxBlock) void bpersistent(BatchParams P, BatchLoop Lp) {
+    __shared__ double red[kMaxBlock * SpmmLanes<R>::Q];
+    __shared__ int flags[2];
+    __shared__ int sflag;
+    const int G = (int)gridDim.x, w = (int)blockIdx.x;
+    unsigned long long target = 0;
+    long long k = 0;
+    while (k < (long long)Lp.rounds) {
+        for (int t = w; t < P.T; t += G) bspmm_dot_tile<VT, R, DIA, WT>(P, t, red);
+        if (!grid_barrier(Lp.bar, target, G, Lp.abortf, &sflag)) break;
+        for (int s = w; s < P.S; s += G) bsys_alpha_sys<R>(P, s, red);
+        if (!grid_barrier(Lp.bar, target, G, Lp.abortf, &sflag)) break;
+        for (int t = w; t < P.T; t += G) bupdate_r_tile<R, WT>(P, t, red);
+        if (!grid_barrier(Lp.bar, target, G, Lp.abortf, &sflag)) break;
+        for (int s = w; s < P.S; s += G) bsys_beta_sys<R>(P, s, red, flags);
+        if (!grid_barrier(Lp.bar, target, G, Lp.abortf, &sflag)) break;
+        for (int t = w; t < P.T; t += G) bupdate_xp_tile<R, WT>(P, t);
+        ++k;
+        if (!grid_barrier(Lp.bar, target, G, Lp.abortf, &sflag)) break;
+        if (!__hip_atomic_load(P.any_active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+    }
+    if (w == 0 && threadIdx.x == 0) {
+        Lp.result[0] = k;
+        Lp.result[1] = __hip_atomic_load(Lp.abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 // --------------------------------------------------------------------------
@@ -590,6 +677,17 @@ static void mask_r(const BatchParams& P, int block, const int* keep, double* v, 
     bmask<R><<<P.T, block, 0, s>>>(P, keep, v);
 }
 
+// the persistent kernel instantiated for (R, value type, layout, vector type)
+template <int R>
+static const void* persistent_fn(bool f32, bool w32, bool dia) {
+    if (w32) {
+        if (dia) return f32 ? (const void*)bpersistent<float, R, true, float> : (const void*)bpersistent<double, R, true, float>;
+        return f32 ? (const void*)bpersistent<float, R, false, float> : (const void*)bpersistent<double, R, false, float>;
+    }
+    if (dia) return f32 ? (const void*)bpersistent<float, R, true, double> : (const void*)bpersistent<double, R, true, double>;
+    return f32 ? (const void*)bpersistent<float, R, false, double> : (const void*)bpersistent<double, R, false, double>;
+}
+
 #define XJ_DISPATCH_R(R_, CALL)                       \
     switch (R_) {                                     \
         case 1: { constexpr int RR = 1; CALL; break; }  \
@@ -611,6 +709,31 @@ hipError_t batch_launch_iteration(const BatchParams& P, int R, bool f32, bool w3
     return hipGetLastError();
 }
 
+
+// Largest co-resident grid of the persistent kernel for this instantiation
+// and block size, one workgroup per CU below the occupancy limit (a grid
+// that fills every slot cannot be placed while any other dispatch holds one).
+hipError_t batch_persistent_grid(int R, bool f32, bool w32, bool dia, int block, int* G) {
+    const void* fn = nullptr;
+    XJ_DISPATCH_R(R, fn = persistent_fn<RR>(f32, w32, dia));
+    int dev = 0, cus = 0, per = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, block, 0);
+    if (e != hipSuccess) return e;
+    *G = cus * (per > 1 ? per - 1 : 1);
+    return hipSuccess;
+}
+
+hipError_t batch_launch_persistent(const BatchParams& P0, const BatchLoop& L0, int R, bool f32, bool w32, int block,
+                                   int G, hipStream_t s) {
+    BatchParams P = P0;
+    BatchLoop L = L0;
+    void* args[] = {&P, &L};
+    const void* fn = nullptr;
+    XJ_DISPATCH_R(R, fn = persistent_fn<RR>(f32, w32, P.dia_on != 0));
+    return hipLaunchCooperativeKernel(fn, dim3(G), dim3(block), args, 0, s);
+}
 
 hipError_t batch_launch_spmm(const BatchParams& P, int R, bool f32, int block, const double* x, double* y,
                              const double* b, hipStream_t s) {

@@ -46,12 +46,28 @@ struct BatchParams {
     uint32_t max_rounds = 0;
 };
 
+// State of the persistent batched loop (one cooperative launch for all
+// iterations): a zeroed grid-barrier arrival counter, a zeroed abort flag,
+// result[0] = iterations run, result[1] = 1 when a barrier timed out.
+struct BatchLoop {
+    unsigned long long* bar = nullptr;
+    int* abortf = nullptr;
+    long long* result = nullptr;
+    uint32_t rounds = 0;
+};
+
 // w32: the work vectors r, p, Ap (P.r, P.p, P.Ap) hold fp32 values (x,
 // arithmetic and dot products stay fp64)
 hipError_t batch_launch_setup(const BatchParams& P, int R, bool w32, int block, hipStream_t s);
 hipError_t batch_launch_iteration(const BatchParams& P, int R, bool f32, bool w32, int block, hipStream_t s);
 hipError_t batch_launch_spmm(const BatchParams& P, int R, bool f32, int block, const double* x, double* y,
                              const double* b, hipStream_t s);
+// the iteration loop as one cooperative launch of G workgroups of `block`
+// threads (G at most batch_persistent_grid's)
+hipError_t batch_persistent_grid(int R, bool f32, bool w32, bool dia, int block, int* G);
+hipError_t batch_launch_persistent(const BatchParams& P, const BatchLoop& L, int R, bool f32, bool w32, int block,
+                                   int G, hipStream_t s);
+
 hipError_t batch_launch_norms(const BatchParams& P, int R, int block, const double* v, double* part, double* out,
                               hipStream_t s);
 hipError_t batch_launch_mask(const BatchParams& P, int R, int block, const int* keep, double* v, hipStream_t s);

@@ -259,6 +259,18 @@ class Batch {
         BatchParams P = params(diag, b, x, tol, rounds);
         hip_check(hipMemsetAsync(any_.ptr, 0, sizeof(int), work_), "reset flag");
         hip_check(batch_launch_setup(P, R_, w32_, block_, work_), "batch set-up");
+        if (path_ == kPathPersistent) {
+            int64_t launched = 0;
+            if (solve_persistent(P, rounds, &launched)) {
+                last_path_ = kPathPersistent;
+                hip_check(batch_launch_finish(S_ * R_, dot_.ptr, rr_.ptr, iters_.ptr, d_iters, d_res, work_),
+                          "finish");
+                join_user(user);
+                hip_check(hipStreamSynchronize(work_), "batch solve");
+                return launched;
+            }
+        }
+        last_path_ = kPathLaunches;
         // iterations run as replays of a captured graph of kChunk iterations
         if (!gexec_ || g_x_ != x || g_rounds_ != rounds) build_graph(P);
         int64_t launched = 0;
@@ -289,6 +301,17 @@ class Batch {
         hip_check(hipStreamSynchronize(work_), "batch solve");
         return launched;
     }
+
+    // 0 / 2: iterations as graph-replayed launches (the default); 1: one
+    // persistent cooperative launch of `grid` workgroups (0: the default size)
+    void set_path(int path, int grid) {
+        if (path < 0 || path > 2) throw invalid("path must be 0 (auto), 1 (persistent) or 2 (launches).");
+        if (grid < 0) throw invalid("grid must be 0 (default) or a positive number of workgroups.");
+        path_ = path == kPathPersistent ? kPathPersistent : kPathLaunches;
+        grid_ = grid;
+    }
+    int last_path() const { return last_path_; }
+    int last_grid() const { return last_grid_; }
 
     // 64: work vectors in fp64 (the default); 32: r, p and Ap stored in fp32
     // with x, the arithmetic and the dot products in fp64 (the inner batch of
@@ -339,6 +362,42 @@ class Batch {
 
   private:
     static constexpr int kChunk = 16;
+    static constexpr int kPathPersistent = 1, kPathLaunches = 2;
+
+    // The iteration loop as one cooperative launch (set-up already queued on
+    // work_).  Returns false, with nothing launched, when the runtime refuses
+    // the cooperative launch (the caller then runs the launch-per-step loop);
+    // throws when a grid barrier timed out.
+    bool solve_persistent(const BatchParams& P, uint32_t rounds, int64_t* launched) {
+        int gmax = 0;
+        hip_check(batch_persistent_grid(R_, f32_, w32_, dia_on_, block_, &gmax), "persistent batch occupancy");
+        int G = std::min(gmax, std::max(T_, S_));
+        if (grid_ > 0) G = std::min(G, grid_);
+        if (G < 1) return false;
+        loop_bar_.ensure(1);
+        loop_abort_.ensure(1);
+        loop_res_.ensure(2);
+        hip_check(hipMemsetAsync(loop_bar_.ptr, 0, sizeof(unsigned long long), work_), "reset barrier");
+        hip_check(hipMemsetAsync(loop_abort_.ptr, 0, sizeof(int), work_), "reset abort flag");
+        BatchLoop L;
+        L.bar = loop_bar_.ptr;
+        L.abortf = loop_abort_.ptr;
+        L.result = loop_res_.ptr;
+        L.rounds = rounds;
+        const hipError_t e = batch_launch_persistent(P, L, R_, f32_, w32_, block_, G, work_);
+        if (e == hipErrorCooperativeLaunchTooLarge || e == hipErrorNotSupported) {
+            (void)hipGetLastError();
+            return false;
+        }
+        hip_check(e, "persistent batch launch");
+        long long res[2] = {0, 0};
+        hip_check(hipMemcpyAsync(res, loop_res_.ptr, sizeof(res), hipMemcpyDeviceToHost, work_), "loop result");
+        hip_check(hipStreamSynchronize(work_), "persistent batch");
+        if (res[1]) throw Error(XJPCG_STATUS_INTERNAL_ERROR, "grid barrier timed out; batched solve aborted.");
+        last_grid_ = G;
+        *launched = res[0];
+        return true;
+    }
 
     BatchParams params(const double* diag, const double* b, double* x, double tol, uint32_t rounds) {
         BatchParams P = P_;
@@ -514,6 +573,10 @@ class Batch {
     DevBuf<double> dot_, tols_, rz_, rr_, alpha_, beta_;
     DevBuf<int> active_, upd_, iters_, it_, sys_active_, sys_upd_, any_;
     DevBuf<double> aux_[3];
+    int path_ = kPathLaunches, grid_ = 0, last_path_ = 0, last_grid_ = 0;
+    DevBuf<unsigned long long> loop_bar_;
+    DevBuf<int> loop_abort_;
+    DevBuf<long long> loop_res_;
 };
 
 }  // namespace xj
@@ -609,6 +672,16 @@ extern "C" XJPCG_Status_t xjpcg_batch_solve(xjpcg_batch_t* batch, const double* 
         int64_t n = impl_of(batch).solve(d_diag, d_b, d_x, max_iter, tol, d_iters, d_res,
                                          static_cast<hipStream_t>(stream));
         if (h_launched) *h_launched = n;
+    });
+}
+
+extern "C" XJPCG_Status_t xjpcg_batch_set_path(xjpcg_batch_t* batch, int32_t path, int32_t grid, int32_t* h_last,
+                                               int32_t* h_grid) {
+    return bguard([&] {
+        xj::Batch& B = impl_of(batch);
+        if (path >= 0) B.set_path(path, grid);
+        if (h_last) *h_last = B.last_path();
+        if (h_grid) *h_grid = B.last_grid();
     });
 }
 

@@ -66,6 +66,23 @@ XJPCG_Status_t xjpcg_batch_solve(xjpcg_batch_t* batch, const double* d_diag, con
                                  uint32_t max_iter, double tol, int32_t* d_iters, double* d_res,
                                  int64_t* h_launched, void* stream);
 
+/* Iteration loop of xjpcg_batch_solve: path 0 (auto) or 2 = graph-replayed
+ * launches, five per batched iteration (SpMM + p.Ap partials, alpha per
+ * pair, r update + r.r / r.z partials, beta / stop rule / masks per pair,
+ * x / p update), kernel boundaries carrying the data between them; path 1 =
+ * one persistent cooperative launch for all iterations, the same five steps
+ * separated by grid barriers, with `grid` co-resident workgroups (0: the
+ * largest grid that stays placeable, at most one workgroup per tile).  Both
+ * produce the same bits; converged systems and pairs drop out through the
+ * same per-system and per-pair masks.  A runtime that refuses the
+ * cooperative launch gets path 2; a grid barrier that waits longer than 5 s
+ * aborts the solve with XJPCG_STATUS_INTERNAL_ERROR.  path < 0 only queries:
+ * *h_last (may be NULL) = the loop the last solve used (0 before any solve),
+ * *h_grid (may be NULL) = its persistent grid.  The reference's control unit
+ * runs its whole loop from one launch (pcg/sw/src/cgHost.cpp:447-471). */
+XJPCG_Status_t xjpcg_batch_set_path(xjpcg_batch_t* batch, int32_t path, int32_t grid, int32_t* h_last,
+                                    int32_t* h_grid);
+
 /* Storage of the work vectors r, p and Ap of xjpcg_batch_solve: bits = 64
  * (the default) or 32.  With 32 they are stored in fp32 -- every value is
  * widened to fp64 before it is used; x, all arithmetic, the per-pair p.Ap,

@@ -221,3 +221,110 @@ def test_batched_mixed_fp32_vectors_matches_oracle(cuda):
     x2, i2 = B64.solve(diag, _dev(cuda, b), 5000, tol)
     assert np.array_equal(x1.cpu().numpy(), x2.cpu().numpy())
     assert np.array_equal(i1.iterations.cpu().numpy(), i2.iterations.cpu().numpy())
+
+
+# ----------------------------------------------------------------------------- one persistent launch
+
+def _both_paths(cuda, B, diag, b, maxit, tol, grid=0):
+    out = {}
+    for path in (2, 1):
+        B.set_path(path, grid)
+        x, info = B.solve(diag, b, maxit, tol)
+        out[path] = (x.cpu().numpy(), info.iterations.cpu().numpy(), info.residual.cpu().numpy(), info.launched)
+        assert B.last_path[0] == path
+    return out
+
+
+@pytest.mark.parametrize("nrhs,split,w32,layout,block,grid", [
+    (4, False, False, 1, 256, 0),      # CSR, fp64
+    (4, False, False, 2, 256, 8),      # diagonals, a grid of 8 workgroups sweeping many tiles each
+    (16, True, False, 2, 512, 0),      # fp32 values + fp64 split diagonal, 16 right-hand sides
+    (8, True, True, 1, 128, 24),       # fp32 values and fp32 work vectors
+    (1, False, False, 2, 64, 0),       # one right-hand side, 64-thread workgroups
+], ids=["csr-fp64", "dia-grid8", "dia-split-r16", "csr-w32-b128", "r1-b64"])
+def test_batched_persistent_same_bits_as_launches(cuda, nrhs, split, w32, layout, block, grid):
+    """The iteration loop as one cooperative launch gives the launch-per-step loop's bits: x, every pair's
+    iteration count and recursive residual, for every value / vector type, layout, block size and grid size;
+    its iteration count is the last pair's (the launches round up to their replay chunk)."""
+    from hpc_amd import batched
+    mats, rhs = small_batch(count=7, nrhs=nrhs, seed=5)
+    asm = batched.assemble(mats, split_fp32=split)
+    B = batched.Batch.from_host(asm, nrhs, cuda)
+    B.configure(tile_rows=128, block=block)
+    assert B.set_layout(layout) == layout
+    if w32:
+        B.set_vectors(32)
+    diag, b = _dev(cuda, asm["diag"]), _dev(cuda, np.concatenate(rhs))
+    tol = 1e-5 if w32 else 1e-8
+    out = _both_paths(cuda, B, diag, b, 5000, tol, grid)
+    its = out[1][1]
+    _say(f"persistent: grid {B.last_path[1]}, {out[1][3]} iterations (launches: {out[2][3]}), pair iterations "
+         f"{its.min()}..{its.max()}")
+    assert np.array_equal(out[1][0], out[2][0])
+    assert np.array_equal(out[1][1], out[2][1])
+    assert np.array_equal(out[1][2], out[2][2])
+    assert out[1][3] == its.max() and out[2][3] >= its.max()
+    assert np.all(out[1][2] < tol)
+    if grid:
+        assert B.last_path[1] == grid
+
+
+def test_batched_persistent_cap_zero_rhs_and_oracle(cuda):
+    """Persistent loop: the maxIter+1 cap ends the launch, a zero right-hand side stays x = 0 without running,
+    and every pair matches the oracle's JPCG (iterations +-1, relative error <= 1e-10)."""
+    from hpc_amd import batched
+    mats, rhs = small_batch(count=4, nrhs=2, seed=9)
+    rhs[2][:, 1] = 0.0
+    asm = batched.assemble(mats)
+    B = batched.Batch.from_host(asm, 2, cuda)
+    B.set_path(1)
+    diag, b = _dev(cuda, asm["diag"]), _dev(cuda, np.concatenate(rhs))
+    x, info = B.solve(diag, b, 7, 1e-14)
+    assert info.launched == 8 and B.last_path[0] == 1
+    its = info.iterations.cpu().numpy()
+    assert its[2, 1] == 0 and np.all(np.delete(its.ravel(), 2 * 2 + 1) == 8)
+    tol, maxit = 1e-9, 5000
+    x, info = B.solve(diag, b, maxit, tol)
+    xh, its = x.cpu().numpy(), info.iterations.cpu().numpy()
+    assert not xh[asm["sys_row"][2]:asm["sys_row"][3], 1].any() and its[2, 1] == 0
+    for s, A in enumerate(mats):
+        r0, r1 = asm["sys_row"][s], asm["sys_row"][s + 1]
+        for j in range(2):
+            if s == 2 and j == 1:
+                continue
+            ref = oracle.jpcg(A.rowptr, A.col, A.val, A.diag, rhs[s][:, j].copy(), maxit, tol)
+            assert abs(int(its[s, j]) - ref["executed"]) <= 1, (s, j)
+            e = np.linalg.norm(xh[r0:r1, j] - ref["x"]) / np.linalg.norm(ref["x"])
+            assert e <= 1e-10, (s, j, e)
+
+
+def test_batched_persistent_inner_of_refinement(cuda):
+    """The inner batch of a refinement on the persistent loop: the same refined solution, bit for bit, as
+    with graph-replayed launches."""
+    from hpc_amd import batched
+    mats, rhs = small_batch(count=5, nrhs=4, seed=8)
+    asm32 = batched.assemble(mats, split_fp32=True)
+    asm64 = batched.assemble(mats)
+    outer = batched.Batch.from_host(asm32, 4, cuda)
+    inner = batched.Batch.from_host(asm32, 4, cuda)
+    inner.set_vectors(32)
+    b, d = _dev(cuda, np.concatenate(rhs)), _dev(cuda, asm64["diag"])
+    res = {}
+    for path in (2, 1):
+        inner.set_path(path)
+        x, info, steps = batched.solve_mixed(outer, inner, d, b, 1e-8, inner_tol=0.0)
+        res[path] = (x.cpu().numpy(), steps, info.residual.cpu().numpy())
+        assert inner.last_path[0] == path
+    assert res[1][1] == res[2][1] and np.array_equal(res[1][0], res[2][0])
+    assert np.all(res[1][2] < 1e-8)
+
+
+def test_batched_path_arguments(cuda):
+    from hpc_amd import batched
+    from hpc_amd._lib import XjpcgError
+    mats, rhs = small_batch(count=2, nrhs=1)
+    B = batched.Batch.from_host(batched.assemble(mats), 1, cuda)
+    assert B.last_path == (0, 0)
+    for bad in ((3, 0), (1, -1)):
+        with pytest.raises(XjpcgError):
+            B.set_path(*bad)

@@ -131,6 +131,17 @@ def test_batch512x16_all_pairs_and_sampled_oracle(cuda):
     x, info = B64.solve(diag, b, maxit, tol)
     torch.cuda.synchronize()
     t64 = time.time() - t0
+    # the same solve as one persistent launch: the same bits
+    B64.set_path(1)
+    t0 = time.time()
+    xp, pinfo = B64.solve(diag, b, maxit, tol)
+    torch.cuda.synchronize()
+    tp = time.time() - t0
+    _say(f"batch: fp64 launches {t64:.2f}s, one persistent launch {tp:.2f}s (grid {B64.last_path[1]}, "
+         f"{pinfo.launched} iterations)")
+    assert B64.last_path[0] == 1
+    assert bool((xp == x).all()) and bool((pinfo.iterations == info.iterations).all())
+    del xp
     t0 = time.time()
     xm, minfo, steps = batched.solve_mixed(outer, inner, diag, b, tol, inner_tol=0.0)
     torch.cuda.synchronize()
