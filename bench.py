@@ -374,6 +374,16 @@ def bench_batched(args):
     its = info.iterations.cpu().numpy()
     log(f"fp64 batch: {np.median(ms64):.1f} ms, {info.launched} launched iterations, pair iterations "
         f"{its.min()}..{its.max()}, max res {float(info.residual.max()):.2e}")
+    # the same solve as one persistent cooperative launch (same bits)
+    B64.set_path(1)
+    xp = torch.empty_like(b)
+    msp, (xp, pinfo) = _timed(lambda: B64.solve(diag, b, maxit, tol, x=xp), 1, 0, stream)
+    persistent = {"ms": round(float(np.median(msp)), 2), "loop": B64.last_path[0], "grid": B64.last_path[1],
+                  "iterations": pinfo.launched,
+                  "same_bits_as_launches": bool((xp == x).all()) and bool((pinfo.iterations == info.iterations).all())}
+    B64.set_path(0)
+    del xp
+    log(f"fp64 batch, one persistent launch: {persistent}")
     xm = torch.empty_like(b)
     msm, (xm, minfo, steps_m) = _timed(lambda: batched.solve_mixed(outer, inner, diag, b, tol,
                                                                    inner_tol=args.inner_tol, x=xm),
@@ -422,6 +432,7 @@ def bench_batched(args):
            "mixed": {"ms": round(float(np.median(msm)), 2), "outer": steps_m, "launched_iterations": minfo.launched,
                      "inner_tol": args.inner_tol, "max_true_res": float(minfo.residual.max()),
                      "config": inner.info()},
+           "fp64_one_persistent_launch": persistent,
            "speedup_mixed_vs_fp64": round(t64 / float(np.median(msm)), 3),
            "mixed_fp32_vectors": {"ms": round(float(np.median(msw)), 2), "outer": steps_w,
                                   "launched_iterations": winfo.launched,
