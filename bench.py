@@ -518,9 +518,24 @@ def bench_autograd(args):
         a, c, gv, gb = fwd_bwd()
         f.append(a)
         bw.append(c)
+    # the gradient kernel on its own: g_k = -lambda_row(k) * x_col(k) over the pattern, 100 launches
+    from hpc_amd import device
+    op = device.CsrOperator(torch.from_numpy(A.rowptr).to(dev), torch.from_numpy(A.col).to(dev), base,
+                            rowptr_host=A.rowptr)
+    lam = torch.from_numpy(workloads.uniform(4, 3, A.n, -1, 1)).to(dev)
+    xs = torch.from_numpy(workloads.uniform(4, 4, A.n, -1, 1)).to(dev)
+    g = torch.empty(A.nnz, dtype=torch.float64, device=dev)
+    sd, _ = _timed(lambda: op.sddmm(lam, xs, g), 100, 5, stream)
+    sd_us = float(np.median(sd)) * 1e3
+    sd_bytes = 12 * A.nnz + 4 * (A.n + 1) + 16 * A.n
+    del op
     out = {"workload": f"2D variable-coefficient diffusion 512^2 (n={A.n}, nnz={A.nnz}), exp(GRF) conductivities "
                        f"(length 16, variance 1, seed 4), tol {tol}",
            "steps": steps, "forward_ms": round(float(np.median(f)), 3), "backward_ms": round(float(np.median(bw)), 3),
+           "sddmm": {"median_us": round(sd_us, 2), "bytes": sd_bytes,
+                     "GBs": round(sd_bytes / (sd_us * 1e-6) / 1e9, 1),
+                     "bytes_basis": "12 nnz + 4(n+1) + 16 n (row pointers, columns, gradient out, lambda and x); "
+                                    "21 MB stays in the 256 MB MALL, so the rate is not an HBM figure"},
            "grad_values_norm": float(gv.norm()), "grad_b_norm": float(gb.norm())}
     log(f"autograd: {out}")
     return out
