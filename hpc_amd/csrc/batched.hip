@@ -294,7 +294,7 @@ __device__ __forceinline__ void bspmm_dot_tile(const BatchParams& P, int t, doub
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const WT a = (WT)acc[q];
-            Ap[(size_t)i * R + q0 + q] = a;
+            st_stream(Ap + (size_t)i * R + q0 + q, a);
             dl[q] += pi[q] * (double)a;
         }
     }
@@ -338,7 +338,7 @@ __global__ __launch_bounds__(kMaxBlock) void bspmm(BatchParams P, const double* 
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const size_t e = (size_t)i * R + q0 + q;
-            y[e] = b ? b[e] - acc[q] : acc[q];
+            st_stream(y + e, b ? b[e] - acc[q] : acc[q]);
         }
     }
 }
@@ -396,7 +396,7 @@ __device__ __forceinline__ void bupdate_r_tile(const BatchParams& P, int t, doub
             if (i + u * G < r1) {
                 const size_t e = (size_t)(i + u * G) * R + j;
                 const double ri = (double)(WT)__builtin_fma(-alpha, av[u], rv[u]);
-                r[e] = (WT)ri;
+                st_stream(r + e, (WT)ri);
                 const double zi = Jv[u] * ri;
                 rr = __builtin_fma(ri, ri, rr);
                 rz = __builtin_fma(ri, zi, rz);
@@ -507,8 +507,8 @@ __device__ __forceinline__ void bupdate_xp_tile(const BatchParams& P, int t) {
         for (int u = 0; u < kRowsAhead; ++u) {
             if (i + u * G < r1) {
                 const size_t e = (size_t)(i + u * G) * R + j;
-                P.x[e] = __builtin_fma(alpha, pv[u], xv[u]);
-                p[e] = (WT)__builtin_fma(beta, pv[u], Jv[u] * rv[u]);
+                st_stream(P.x + e, __builtin_fma(alpha, pv[u], xv[u]));
+                st_stream(p + e, (WT)__builtin_fma(beta, pv[u], Jv[u] * rv[u]));
             }
         }
     }

@@ -193,6 +193,15 @@ __device__ __forceinline__ T ld_stream(const T* p) {
     return __builtin_nontemporal_load(p);
 }
 
+// A streamed output (y = A x, Ap, the vector updates' r, x, p): written once
+// and read again only by a later kernel, so it is stored without allocating
+// in L2, where it would evict lines the launch still reuses (x of an SpMV).
+// The stored value is the same; only the cache policy differs.
+template <typename T>
+__device__ __forceinline__ void st_stream(T* p, T v) {
+    __builtin_nontemporal_store(v, p);
+}
+
 // ---------------------------------------------------------------------------
 // CSR tile SpMV.
 //
