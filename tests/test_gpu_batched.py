@@ -328,3 +328,35 @@ def test_batched_path_arguments(cuda):
     for bad in ((3, 0), (1, -1)):
         with pytest.raises(XjpcgError):
             B.set_path(*bad)
+
+
+@pytest.mark.parametrize("nrhs", [2, 32])
+def test_batched_persistent_more_rhs_and_oversized_grid(cuda, nrhs):
+    """R = 2 and R = 32 on the persistent loop, with a grid request far above the tile count (capped to what
+    the batch can use): the launch loop's bits."""
+    from hpc_amd import batched
+    mats, rhs = small_batch(count=3, nrhs=nrhs, nmin=500, nmax=2000, seed=13)
+    asm = batched.assemble(mats)
+    B = batched.Batch.from_host(asm, nrhs, cuda)
+    diag, b = _dev(cuda, asm["diag"]), _dev(cuda, np.concatenate(rhs))
+    out = _both_paths(cuda, B, diag, b, 5000, 1e-8, grid=100000)
+    _say(f"R={nrhs}: persistent grid {B.last_path[1]} for {B.info()['tiles']} tiles")
+    assert 0 < B.last_path[1] <= max(B.info()["tiles"], B.S)
+    assert np.array_equal(out[1][0], out[2][0]) and np.array_equal(out[1][1], out[2][1])
+    assert np.all(out[1][2] < 1e-8)
+
+
+def test_batched_persistent_all_zero_rhs(cuda):
+    """No pair runs: the persistent launch leaves after one masked iteration with x = 0 and no pair
+    iterations."""
+    import torch
+    from hpc_amd import batched
+    mats, rhs = small_batch(count=2, nrhs=4)
+    asm = batched.assemble(mats)
+    B = batched.Batch.from_host(asm, 4, cuda)
+    B.set_path(1)
+    b = torch.zeros((B.N, 4), dtype=torch.float64, device=cuda)
+    x, info = B.solve(_dev(cuda, asm["diag"]), b, 100, 1e-8)
+    assert B.last_path[0] == 1 and info.launched == 1
+    assert not x.cpu().numpy().any() and not info.iterations.cpu().numpy().any()
+    assert not info.residual.cpu().numpy().any()
