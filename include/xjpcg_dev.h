@@ -58,11 +58,15 @@ XJPCG_Status_t xjpcg_op_set_compact(xjpcg_op_t* op, int on, int* h_active);
 XJPCG_Status_t xjpcg_op_set_values(xjpcg_op_t* op, const void* d_val);
 /* SpMV layout (no counterpart in the reference, whose SpMV streams its own
  * partitioned format from HBM): 1 = row tiles over the CSR arrays (int16 or
- * int32 column stream, see xjpcg_op_set_compact);
+ * int32 column stream, see xjpcg_op_set_compact); rows of up to 64 entries
+ * are summed left to right by one lane, longer rows by a wavefront in a fixed
+ * order (strided lane sums, xor butterfly), rows above a tile's 2044 entries
+ * by the workgroup;
  * 2 = panel-swept copy for matrices whose columns scatter over x (row blocks
  * of up to 4096 rows keep their sums in LDS and sweep 65536-column panels of
  * x in order, so the gathered window stays in L2; 12 bytes per entry plus an
- * 8-byte value map; rows are still summed left to right);
+ * 8-byte value map; a row's run inside a staged chunk is added left to right
+ * when shorter than 15 entries, as one wavefront total otherwise);
  * 3 = row tiles on column-offset patterns (stencil operators: each tile's at
  * most 32 distinct offsets col - row plus a 32-bit mask per row replace the
  * column stream, 4 bytes per row instead of 2-4 per entry; same sums as 1);
