@@ -47,13 +47,16 @@ uint32_t validate_cscsym(uint32_t n, uint32_t nnz_half, const int64_t* row_idx, 
                          bool* standard);
 
 // Device builds into caller device arrays d_rowptr[n+1], d_col[m], d_perm[m]
-// (m = nnz, or the expanded count for CSC-sym); validate first, then build
-// on the GPU (CSC-sym with a non-standard col_ptr: on the host, then upload).
-// The CSR equals the host build's.  Synchronises `s`.
+// (m = nnz, or the expanded count for CSC-sym); validate first (unless the
+// caller already ran validate_coo / validate_cscsym on the same arrays:
+// validated = true), then build on the GPU (CSC-sym with a non-standard
+// col_ptr: on the host, then upload).  The CSR equals the host build's.
+// Synchronises `s`.
 void csr_from_coo_device(uint32_t n, uint32_t nnz, const uint32_t* row, const uint32_t* col, int base,
-                         int32_t* d_rowptr, int32_t* d_col, int64_t* d_perm, hipStream_t s);
+                         int32_t* d_rowptr, int32_t* d_col, int64_t* d_perm, hipStream_t s, bool validated = false);
 uint32_t csr_from_cscsym_device(uint32_t n, uint32_t nnz_half, const int64_t* row_idx, const int64_t* col_ptr,
-                                int base, int32_t* d_rowptr, int32_t* d_col, int64_t* d_perm, hipStream_t s);
+                                int base, int32_t* d_rowptr, int32_t* d_col, int64_t* d_perm, hipStream_t s,
+                                bool validated = false);
 
 // kernels and hipCUB passes (ingest.hip)
 hipError_t ingest_coo_keys(int64_t m, uint32_t n, uint32_t base, const uint32_t* row, const uint32_t* col,

@@ -94,7 +94,7 @@ class PcgHandle {
         validate_coo(n, nnz, row, col, base);  // every error before the device is touched
         require_device();
         build_on_device(n, nnz, nnz, data, [&](int32_t* rp, int32_t* c, int64_t* pm) {
-            csr_from_coo_device(n, nnz, row, col, base, rp, c, pm, stream_);
+            csr_from_coo_device(n, nnz, row, col, base, rp, c, pm, stream_, true);
         });
     }
 
@@ -125,7 +125,7 @@ class PcgHandle {
         const uint32_t full = validate_cscsym(n, nnz_half, row_idx, col_ptr, base, nullptr);
         require_device();
         build_on_device(n, full, nnz_half, data, [&](int32_t* rp, int32_t* c, int64_t* pm) {
-            csr_from_cscsym_device(n, nnz_half, row_idx, col_ptr, base, rp, c, pm, stream_);
+            csr_from_cscsym_device(n, nnz_half, row_idx, col_ptr, base, rp, c, pm, stream_, true);
         });
     }
 

@@ -124,6 +124,32 @@ def test_pressure_sequence():
         assert h.last_status == Status.SUCCESS and m.m_objInit >= 0.0
 
 
+def test_large_coo_reports_first_bad_entry():
+    """Index checks of a COO too large for one walking thread (split over host threads): the first
+    out-of-range entry in input order is the one reported, as the serial walk of the reference loader stops
+    there, and no device is needed to reach it."""
+    n, nnz = 1000, 9_000_000
+    rng = np.random.default_rng(4)
+    row = rng.integers(0, n, nnz).astype(np.uint32)
+    col = rng.integers(0, n, nnz).astype(np.uint32)
+    val = np.ones(nnz)
+    diag, b, x = np.ones(n), np.ones(n), np.zeros(n)
+    with JPCG() as h:
+        for bads in ([8_999_999], [6_100_000, 2_300_001], [4_500_000, 4_500_001, 1], [0, 8_000_000]):
+            c = col.copy()
+            for k in bads:
+                c[k] = n + k % 7
+            assert h.coo_solve_raw(n, nnz, row, c, val, diag, b, x, 10, 1e-8) == Status.INVALID_VALUE
+            assert h.last_message == f"INVALID VALUE ERROR: matrix index out of range at entry {min(bads)}.", bads
+        # Fortran base: a zero index underflows
+        r1 = row + 1
+        c1 = col + 1
+        r1[7_777_777] = 0
+        assert h.coo_solve_raw(n, nnz, r1, c1, val, diag, b, x, 10, 1e-8, Mode.FORTRAN_INDEX) == \
+            Status.INVALID_VALUE
+        assert h.last_message == "INVALID VALUE ERROR: matrix index out of range at entry 7777777."
+
+
 def test_cscsym_argument_errors():
     n = 3
     with JPCG() as h:
