@@ -1,5 +1,6 @@
 // Matrix ingest on the host and on the GPU (see ingest.hpp).
 #include "ingest.hpp"
+#include "staging.hpp"
 
 #include <algorithm>
 #include <thread>
@@ -217,9 +218,9 @@ void csr_from_coo_device(uint32_t n, uint32_t nnz, const uint32_t* row, const ui
     const size_t tb = ingest_sort_temp_bytes(m, n);
     temp.ensure(std::max<size_t>(tb, 1));
     if (m > 0) {
-        hip_check(hipMemcpyAsync(drow.ptr, row, sizeof(uint32_t) * (size_t)m, hipMemcpyHostToDevice, s), "upload rows");
-        hip_check(hipMemcpyAsync(dcol.ptr, col, sizeof(uint32_t) * (size_t)m, hipMemcpyHostToDevice, s),
-                  "upload columns");
+        const HostPiece up[2] = {{drow.ptr, row, sizeof(uint32_t) * (size_t)m},
+                                 {dcol.ptr, col, sizeof(uint32_t) * (size_t)m}};
+        staged_upload(up, 2, s);
     }
     hip_check(ingest_coo_keys(m, n, (uint32_t)base, drow.ptr, dcol.ptr, key.ptr, idx.ptr, s), "COO keys");
     hip_check(ingest_sort_build(m, n, key.ptr, idx.ptr, key2.ptr, idx2.ptr, temp.ptr, tb, d_rowptr, d_col, d_perm, s),
@@ -242,15 +243,10 @@ uint32_t csr_from_cscsym_device(uint32_t n, uint32_t nnz_half, const int64_t* ro
         // overlapping or unordered column ranges: expand in the loader's own order on the host
         Pattern P;
         pattern_from_cscsym(P, n, nnz_half, row_idx, col_ptr, base);
-        hip_check(hipMemcpyAsync(d_rowptr, P.rowptr.data(), sizeof(int32_t) * P.rowptr.size(), hipMemcpyHostToDevice,
-                                 s),
-                  "upload row pointers");
-        if (P.nnz > 0) {
-            hip_check(hipMemcpyAsync(d_col, P.col.data(), sizeof(int32_t) * P.col.size(), hipMemcpyHostToDevice, s),
-                      "upload columns");
-            hip_check(hipMemcpyAsync(d_perm, P.perm.data(), sizeof(int64_t) * P.perm.size(), hipMemcpyHostToDevice, s),
-                      "upload value map");
-        }
+        const HostPiece up[3] = {{d_rowptr, P.rowptr.data(), sizeof(int32_t) * P.rowptr.size()},
+                                 {d_col, P.col.data(), sizeof(int32_t) * P.col.size()},
+                                 {d_perm, P.perm.data(), sizeof(int64_t) * P.perm.size()}};
+        staged_upload(up, P.nnz > 0 ? 3 : 1, s);
         hip_check(hipStreamSynchronize(s), "CSC-sym ingest");
         return full;
     }
@@ -271,10 +267,9 @@ uint32_t csr_from_cscsym_device(uint32_t n, uint32_t nnz_half, const int64_t* ro
     idx2.ensure((size_t)m);
     const size_t tscan = ingest_scan_temp_bytes(mh), tsort = ingest_sort_temp_bytes(m, n);
     temp.ensure(std::max<size_t>(std::max(tscan, tsort), 1));
-    hip_check(hipMemcpyAsync(dri.ptr, row_idx, sizeof(int64_t) * (size_t)mh, hipMemcpyHostToDevice, s),
-              "upload row indices");
-    hip_check(hipMemcpyAsync(dcp.ptr, col_ptr, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyHostToDevice, s),
-              "upload column pointers");
+    const HostPiece up[2] = {{dri.ptr, row_idx, sizeof(int64_t) * (size_t)mh},
+                             {dcp.ptr, col_ptr, sizeof(int64_t) * ((size_t)n + 1)}};
+    staged_upload(up, 2, s);
     hip_check(ingest_csc_keys(mh, n, base, dri.ptr, dcp.ptr, colof.ptr, cnt.ptr, pos.ptr, temp.ptr, tscan, key.ptr,
                               idx.ptr, s),
               "CSC-sym expansion");

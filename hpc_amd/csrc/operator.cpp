@@ -6,6 +6,7 @@
 #include <cstring>
 
 #include "kernels_consts.hpp"
+#include "staging.hpp"
 
 namespace xj {
 
@@ -486,14 +487,11 @@ bool Operator::build_panels(hipStream_t s, const int32_t* rp_host, const int32_t
         free_panels();  // the panel layout is optional: keep the tiles
         return false;
     }
-    hip_check(hipMemcpyAsync(pseg_.ptr, pl.seg.data(), sizeof(int32_t) * pl.seg.size(), hipMemcpyHostToDevice, s),
-              "upload panel segments");
-    hip_check(hipMemcpyAsync(poff_.ptr, pl.off.data(), sizeof(uint16_t) * pl.off.size(), hipMemcpyHostToDevice, s),
-              "upload panel columns");
-    hip_check(hipMemcpyAsync(plrow_.ptr, pl.lrow.data(), sizeof(uint16_t) * pl.lrow.size(), hipMemcpyHostToDevice, s),
-              "upload panel rows");
-    hip_check(hipMemcpyAsync(pperm_.ptr, pl.perm.data(), sizeof(int64_t) * pl.perm.size(), hipMemcpyHostToDevice, s),
-              "upload panel map");
+    const HostPiece up[4] = {{pseg_.ptr, pl.seg.data(), sizeof(int32_t) * pl.seg.size()},
+                             {poff_.ptr, pl.off.data(), sizeof(uint16_t) * pl.off.size()},
+                             {plrow_.ptr, pl.lrow.data(), sizeof(uint16_t) * pl.lrow.size()},
+                             {pperm_.ptr, pl.perm.data(), sizeof(int64_t) * pl.perm.size()}};
+    staged_upload(up, 4, s);
     A_.pan.nb = pl.nb;
     A_.pan.block_rows = pl.block_rows;
     A_.pan.P = pl.P;

@@ -19,6 +19,7 @@
 #include "ingest.hpp"
 #include "operator.hpp"
 #include "pcg.h"
+#include "staging.hpp"
 #include "xjpcg_ext.h"
 
 namespace xj {
@@ -149,9 +150,8 @@ class PcgHandle {
         d_b_.ensure(n);
         d_diag_.ensure(n);
         d_x_.ensure(n);
-        hip_check(hipMemcpyAsync(d_b_.ptr, b, sizeof(double) * n, hipMemcpyHostToDevice, stream_), "upload b");
-        hip_check(hipMemcpyAsync(d_diag_.ptr, diag, sizeof(double) * n, hipMemcpyHostToDevice, stream_),
-                  "upload diagonal");
+        const HostPiece up[2] = {{d_b_.ptr, b, sizeof(double) * n}, {d_diag_.ptr, diag, sizeof(double) * n}};
+        staged_upload(up, 2, stream_);
         hip_check(hipStreamSynchronize(stream_), "upload vectors");
         vec_n_ = n;
     }
@@ -222,12 +222,10 @@ class PcgHandle {
         d_col_.ensure((size_t)P.nnz);
         d_val_.ensure((size_t)P.nnz);
         d_perm_.ensure((size_t)P.nnz);
-        hip_check(hipMemcpyAsync(d_rowptr_.ptr, P.rowptr.data(), sizeof(int32_t) * ((size_t)n + 1),
-                                 hipMemcpyHostToDevice, stream_), "upload row pointers");
-        hip_check(hipMemcpyAsync(d_col_.ptr, P.col.data(), sizeof(int32_t) * (size_t)P.nnz, hipMemcpyHostToDevice,
-                                 stream_), "upload columns");
-        hip_check(hipMemcpyAsync(d_perm_.ptr, P.perm.data(), sizeof(int64_t) * (size_t)P.nnz, hipMemcpyHostToDevice,
-                                 stream_), "upload value map");
+        const HostPiece up[3] = {{d_rowptr_.ptr, P.rowptr.data(), sizeof(int32_t) * ((size_t)n + 1)},
+                                 {d_col_.ptr, P.col.data(), sizeof(int32_t) * (size_t)P.nnz},
+                                 {d_perm_.ptr, P.perm.data(), sizeof(int64_t) * (size_t)P.nnz}};
+        staged_upload(up, 3, stream_);
         pat_n_ = n;
         pat_nnz_ = P.nnz;
         src_len_ = P.src_len;
@@ -261,8 +259,7 @@ class PcgHandle {
     // values reach their CSR slots on the GPU: val[k] = data[perm[k]]
     void refresh_values(const double* data, int64_t len) {
         d_raw_.ensure((size_t)len);
-        hip_check(hipMemcpyAsync(d_raw_.ptr, data, sizeof(double) * (size_t)len, hipMemcpyHostToDevice, stream_),
-                  "upload values");
+        staged_upload(d_raw_.ptr, data, sizeof(double) * (size_t)len, stream_);
         hip_check(launch_gather_f64(pat_nnz_, d_perm_.ptr, d_raw_.ptr, d_val_.ptr, stream_), "place values");
         hip_check(hipStreamSynchronize(stream_), "place values");
         if (op_) op_->set_values(d_val_.ptr);  // values changed in place: refresh the operator's own copies

@@ -9,6 +9,7 @@
 #include "launch.hpp"
 #include "operator.hpp"
 #include "signature.hpp"
+#include "staging.hpp"
 
 struct xjpcg_sig {
     xj::sig::Image im;
@@ -176,16 +177,14 @@ XJPCG_Status_t xjpcg_sig_to_csr(const xjpcg_sig_t* sig, int32_t* d_rowptr, int32
         cnt.ensure((size_t)vw);
         pos.ensure((size_t)vw);
         bad.ensure(1);
+        std::vector<xj::HostPiece> up;
         size_t off = 0;
         for (const auto& st : im.nnz_val) {
-            xj::hip_check(hipMemcpyAsync(words.ptr + off, st.data(), st.size(), hipMemcpyHostToDevice, s),
-                          "upload streams");
+            up.push_back({words.ptr + off, st.data(), st.size()});
             off += st.size();
         }
-        if (nseg)
-            xj::hip_check(hipMemcpyAsync(dseg.ptr, seg.data(), sizeof(xj::sig::Segment) * nseg, hipMemcpyHostToDevice,
-                                         s),
-                          "upload segments");
+        up.push_back({dseg.ptr, seg.data(), sizeof(xj::sig::Segment) * nseg});
+        xj::staged_upload(up.data(), up.size(), s);
         xj::hip_check(xj::sig::decode_count(vw, dseg.ptr, nseg, words.ptr, cnt.ptr, s), "signature count");
         uint64_t total = 0;
         if (vw > 0) {
