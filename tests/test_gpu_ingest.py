@@ -46,6 +46,18 @@ def test_csr_from_coo_matches_stable_sort(cuda, base):
     assert np.array_equal(pm.cpu().numpy(), rperm)
 
 
+def test_csr_from_coo_staged_chunks(cuda):
+    """Index arrays of 25 MB each: uploaded through the pinned staging pool in 8 MB chunks with a ragged tail."""
+    from hpc_amd import device
+    n, m = 1_000_000, 3 * (1 << 21) + 12_345
+    rows, cols = random_coo(n, m, 17)
+    rp, c, pm = device.csr_from_coo(n, rows, cols, 0, cuda)
+    rrp, rc, rperm = ref_csr(n, rows, cols)
+    assert np.array_equal(rp.cpu().numpy(), rrp)
+    assert np.array_equal(c.cpu().numpy(), rc)
+    assert np.array_equal(pm.cpu().numpy(), rperm)
+
+
 def test_csr_from_coo_empty_rows_and_edges(cuda):
     from hpc_amd import device
     n = 1000
@@ -150,6 +162,24 @@ def test_pcg_cscsym_solver_device_ingest_matches_oracle(cuda):
     with JPCG(0) as h:
         out = h.csc_sym_solve(A.n, r + 1, col_ptr + 1, v, A.diag, s.b, 5000, 1e-12, mode=0x10)
     _say(f"cscsym device ingest: {out.iterations} iterations (oracle {ref['iter']}), status {out.status}")
+    assert out.status == Status.SUCCESS
+    assert abs(out.iterations - ref["iter"]) <= 1
+    assert np.linalg.norm(out.x - ref["x"]) / np.linalg.norm(ref["x"]) <= 1e-10
+
+
+def test_pcg_coo_solver_staged_uploads_match_oracle(cuda):
+    """xJPCG_cooSolver on the 27-point 64^3 operator as a shuffled COO with duplicates (6.3M entries): indices,
+    values (50 MB) and the two vectors (4.2 MB together) go through the pinned staging pool."""
+    from hpc_amd.pcg import JPCG, Status
+    s = workloads.c2_stencil27(64)
+    A = s.A
+    rows, cols, vals = _shuffled_coo(A, 13)
+    assert vals.nbytes > 6 * (8 << 20) and 16 * A.n >= 4 << 20
+    rp, col, val, _ = oracle.coo_to_csr(A.n, rows, cols, vals)
+    ref = oracle.jpcg(rp, col, val, A.diag, s.b, 2000, s.tol, threads=8)
+    with JPCG(0) as h:
+        out = h.coo_solve(A.n, rows, cols, vals, A.diag, s.b, 2000, s.tol)
+    _say(f"64^3 coo staged: {out.iterations} iterations (oracle {ref['iter']}), status {out.status}")
     assert out.status == Status.SUCCESS
     assert abs(out.iterations - ref["iter"]) <= 1
     assert np.linalg.norm(out.x - ref["x"]) / np.linalg.norm(ref["x"]) <= 1e-10
