@@ -73,17 +73,20 @@ def cpu_model():
 
 
 LAYOUT_NAMES = {1: "tiles", 2: "panels", 3: "patterns", 4: "diagonals"}
-LAYOUT_KERNELS = {1: "spmv_tiles<%s, %s>", 2: "spmv_panels<%s>", 3: "spmv_pattern<%s>", 4: "spmv_dia<%s>"}
+LAYOUT_KERNELS = {1: "spmv_tiles<%s, %s>", 2: "spmv_panels<%s>", 3: "spmv_pattern<%s>", 4: "spmv_dia<%s, %d>"}
+DIA_WIDE_ROWS = 1 << 19   # from this many rows the fp64 diagonal kernel takes two rows per lane
 TRAFFIC_PROFILES = {("stencil27_160", "tiles"): "spmv_tiles_stencil27_160.md",
                     ("stencil27_160", "patterns"): "spmv_pattern_stencil27_160.md",
                     ("stencil27_160", "diagonals"): "spmv_dia_stencil27_160.md",
                     ("graph4m", "panels"): "spmv_panels_graph4m.md"}
 
 
-def kernel_name(layout, f32, compact):
+def kernel_name(layout, f32, compact, n):
     vt = "float" if f32 else "double"
     if layout == 1:
         return LAYOUT_KERNELS[1] % (vt, "int16_t" if compact else "int")
+    if layout == 4:
+        return LAYOUT_KERNELS[4] % (vt, 1 if f32 or n < DIA_WIDE_ROWS else 2)
     return LAYOUT_KERNELS[layout] % vt
 
 
@@ -165,7 +168,7 @@ def layout_rates(op, layouts, stream, repeats, csr_nb):
                          "GBs": round(nb / (mean_us * 1e-6) / 1e9, 1),
                          "frac_of_peak": round(nb / (mean_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
                          "csr_equivalent_GBs": round(csr_nb / (mean_us * 1e-6) / 1e9, 1),
-                         "kernel": kernel_name(L, op.f32, bool(compact))}
+                         "kernel": kernel_name(L, op.f32, bool(compact), op.n)}
             if compact is not None:
                 op.set_compact(True)
     op.set_layout(0)
@@ -836,7 +839,7 @@ def bench_headline(args):
     wl = f"stencil27_{args.grid}"
     lname = LAYOUT_NAMES[chosen]
     traffic, traffic_src = measured_traffic(wl, lname)
-    kname = kernel_name(chosen, False, op.compact)
+    kname = kernel_name(chosen, False, op.compact, op.n)
     out = {
         "metric": "JPCG time-to-1e-10 residual & SpMV HBM GB/s: mixed-refined vs fp64, 1 GPU",
         "value": round(spmv_gbs, 1),

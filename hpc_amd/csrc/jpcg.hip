@@ -717,13 +717,13 @@ __global__ __launch_bounds__(kBlock) void spmv_pattern(int ntiles, int nnz, cons
 // operands and has mask 0; callers drop its sum.  dia_rows_s reads x one
 // element per row; dia_rows_v (R > 1, n >= R) reads a diagonal's R x values
 // with one load.
-template <typename VT>
+template <int R>
 struct DiaLane {
-    static constexpr int R = sizeof(VT) == 8 ? 2 : 1;
-    static constexpr int C = 16 / R;
+    static constexpr int C = 16 / R;  // diagonals per batch of loads
 };
-static_assert(DiaLane<double>::R == dia_rows_per_lane(false) && DiaLane<float>::R == dia_rows_per_lane(true),
-              "rows per lane agree with the host's grid");
+static_assert(dia_rows_per_lane(true, kDiaWideRows) == 1 && dia_rows_per_lane(false, kDiaWideRows) == 2 &&
+                  dia_rows_per_lane(false, kDiaWideRows - 1) == 1,
+              "the kernels are instantiated for these rows per lane");
 static_assert(kDiaBlock == kBlock, "diagonal kernels run kBlock lanes");
 static_assert(kVectorBlock == kBlock, "vector kernels run kBlock lanes");
 
@@ -739,11 +739,11 @@ __device__ __forceinline__ void ld_rows(const T* p, T (&o)[R]) {
     }
 }
 
-template <typename VT, typename XT = double>
+template <typename VT, int R, typename XT = double>
 __device__ __forceinline__ void dia_rows_s(const DiaDev& D, int n, int i0, const double* __restrict__ dsplit,
-                                         const XT* __restrict__ x, double (&s)[DiaLane<VT>::R]) {
+                                         const XT* __restrict__ x, double (&s)[R]) {
 #pragma clang fp contract(off)
-    constexpr int R = DiaLane<VT>::R, C = DiaLane<VT>::C;
+    constexpr int C = DiaLane<R>::C;
     const VT* __restrict__ val = static_cast<const VT*>(D.val);
     const int* __restrict__ off = D.off;
     const int nd = D.ndia;
@@ -786,11 +786,11 @@ __device__ __forceinline__ void dia_rows_s(const DiaDev& D, int n, int i0, const
 // The same rows with x read as one R-wide load per diagonal (R > 1, n >= R):
 // the load's start is clamped into [0, n-R]; a column shifted out of [0, n)
 // belongs to a diagonal the row does not have (mask 0).
-template <typename VT>
+template <typename VT, int R>
 __device__ __forceinline__ void dia_rows_v(const DiaDev& D, int n, int i0, const double* __restrict__ dsplit,
-                                           const double* __restrict__ x, double (&s)[DiaLane<VT>::R]) {
+                                           const double* __restrict__ x, double (&s)[R]) {
 #pragma clang fp contract(off)
-    constexpr int R = DiaLane<VT>::R, C = DiaLane<VT>::C;
+    constexpr int C = DiaLane<R>::C;
     typedef double XV __attribute__((ext_vector_type(R), aligned(8)));
     const VT* __restrict__ val = static_cast<const VT*>(D.val);
     const int* __restrict__ off = D.off;
@@ -841,19 +841,18 @@ __device__ __forceinline__ void dia_rows_v(const DiaDev& D, int n, int i0, const
 // 59 (occupancy 5 instead of 8).
 
 // y = A x (or b - A x), R consecutive rows per lane, 256 R rows per workgroup
-template <typename VT>
+template <typename VT, int R>
 __global__ __launch_bounds__(kBlock) void spmv_dia(int n, DiaDev D, const double* __restrict__ dsplit,
                                                    const double* __restrict__ x, double* __restrict__ y,
                                                    const double* __restrict__ b) {
-    constexpr int R = DiaLane<VT>::R;
     const int i0 = ((int)blockIdx.x * kBlock + (int)threadIdx.x) * R;
     if (i0 >= n) return;
     double s[R];
     if constexpr (R > 1) {
-        if (n >= R) dia_rows_v<VT>(D, n, i0, dsplit, x, s);
-        else dia_rows_s<VT>(D, n, i0, dsplit, x, s);
+        if (n >= R) dia_rows_v<VT, R>(D, n, i0, dsplit, x, s);
+        else dia_rows_s<VT, R>(D, n, i0, dsplit, x, s);
     } else {
-        dia_rows_s<VT>(D, n, i0, dsplit, x, s);
+        dia_rows_s<VT, R>(D, n, i0, dsplit, x, s);
     }
 #pragma unroll
     for (int r = 0; r < R; ++r)
@@ -869,9 +868,8 @@ __global__ __launch_bounds__(kBlock) void spmv_dia(int n, DiaDev D, const double
 // then the wavefronts in order -- a fixed order.  With fp32 work vectors (PT =
 // float) p is widened as it is read, Ap is rounded as it is stored and the
 // partial uses the stored value.
-template <typename VT, typename PT>
+template <typename VT, typename PT, int R>
 __global__ __launch_bounds__(kBlock) void cg_spmv_dia(CgParams P) {
-    constexpr int R = DiaLane<VT>::R;
     CgState* st = P.st;
     const int stop = st->stop;
     const int b = (int)blockIdx.x;
@@ -885,10 +883,10 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_dia(CgParams P) {
 #pragma unroll
         for (int r = 0; r < R; ++r) pi[r] = (double)p[i0 + r < P.n ? i0 + r : P.n - 1];
         if constexpr (R > 1 && sizeof(PT) == 8) {
-            if (P.n >= R) dia_rows_v<VT>(P.dia, P.n, i0, P.dsplit, P.p, y);
-            else dia_rows_s<VT>(P.dia, P.n, i0, P.dsplit, P.p, y);
+            if (P.n >= R) dia_rows_v<VT, R>(P.dia, P.n, i0, P.dsplit, P.p, y);
+            else dia_rows_s<VT, R>(P.dia, P.n, i0, P.dsplit, P.p, y);
         } else {
-            dia_rows_s<VT, PT>(P.dia, P.n, i0, P.dsplit, p, y);
+            dia_rows_s<VT, R, PT>(P.dia, P.n, i0, P.dsplit, p, y);
         }
 #pragma unroll
         for (int r = 0; r < R; ++r)
@@ -1211,16 +1209,21 @@ hipError_t launch_cg_iteration(const CgParams& P, bool f32, bool c16, hipStream_
         }
     } else if (P.dia_on) {
         const int g = dia_groups(P.n, f32);
+        const bool wide = dia_rows_per_lane(f32, P.n) == 2;  // fp64 values only
         if (P.w32) {
             if (f32)
-                cg_spmv_dia<float, float><<<g, kBlock, 0, s>>>(P);
+                cg_spmv_dia<float, float, 1><<<g, kBlock, 0, s>>>(P);
+            else if (wide)
+                cg_spmv_dia<double, float, 2><<<g, kBlock, 0, s>>>(P);
             else
-                cg_spmv_dia<double, float><<<g, kBlock, 0, s>>>(P);
+                cg_spmv_dia<double, float, 1><<<g, kBlock, 0, s>>>(P);
         } else {
             if (f32)
-                cg_spmv_dia<float, double><<<g, kBlock, 0, s>>>(P);
+                cg_spmv_dia<float, double, 1><<<g, kBlock, 0, s>>>(P);
+            else if (wide)
+                cg_spmv_dia<double, double, 2><<<g, kBlock, 0, s>>>(P);
             else
-                cg_spmv_dia<double, double><<<g, kBlock, 0, s>>>(P);
+                cg_spmv_dia<double, double, 1><<<g, kBlock, 0, s>>>(P);
         }
     } else if (P.pat_on) {
         if (f32)
@@ -1259,9 +1262,11 @@ hipError_t launch_spmv(const CsrDev& A, const double* x, double* y, const double
         const int g = dia_groups(A.n, A.val_f32);
         if (g <= 0) return hipSuccess;
         if (A.val_f32)
-            spmv_dia<float><<<g, kBlock, 0, s>>>(A.n, A.dia, A.dsplit, x, y, b);
+            spmv_dia<float, 1><<<g, kBlock, 0, s>>>(A.n, A.dia, A.dsplit, x, y, b);
+        else if (dia_rows_per_lane(false, A.n) == 2)
+            spmv_dia<double, 2><<<g, kBlock, 0, s>>>(A.n, A.dia, A.dsplit, x, y, b);
         else
-            spmv_dia<double><<<g, kBlock, 0, s>>>(A.n, A.dia, A.dsplit, x, y, b);
+            spmv_dia<double, 1><<<g, kBlock, 0, s>>>(A.n, A.dia, A.dsplit, x, y, b);
         return hipGetLastError();
     }
     if (A.ntiles <= 0) return hipSuccess;

@@ -35,8 +35,11 @@ struct PanelDev {
 // row has no entry there, and a presence mask per row.
 constexpr int kDiaMax = 32;
 // Rows per lane of the diagonal kernels: two consecutive rows for fp64 values
-// (one 16-byte load per diagonal), one row for fp32 values.
-constexpr int dia_rows_per_lane(bool f32) { return f32 ? 1 : 2; }
+// (one 16-byte load per diagonal) from kDiaWideRows rows up, one row for fp32
+// values and for smaller fp64 operators (two rows per lane would leave a
+// small operator's grid at a workgroup or two per CU).
+constexpr int kDiaWideRows = 1 << 19;
+constexpr int dia_rows_per_lane(bool f32, int n) { return f32 || n < kDiaWideRows ? 1 : 2; }
 // Leading dimension of the diagonal value arrays and length of the mask
 // array: n rounded up to even, so a lane's two rows are one aligned 16-byte
 // load; the padding row's mask is 0 and its values are 0.
@@ -45,7 +48,7 @@ constexpr int dia_ld(int n) { return (n + 1) & ~1; }
 // SpMV's p.Ap partials (one per workgroup)
 constexpr int kDiaBlock = 256;
 constexpr int dia_groups(int n, bool f32) {
-    return (n + kDiaBlock * dia_rows_per_lane(f32) - 1) / (kDiaBlock * dia_rows_per_lane(f32));
+    return (n + kDiaBlock * dia_rows_per_lane(f32, n) - 1) / (kDiaBlock * dia_rows_per_lane(f32, n));
 }
 constexpr int dia_cg_partials(int n, bool f32) { return dia_groups(n, f32); }
 // Workgroups of the CG vector kernels (256 lanes, grid-stride; one r.r / r.z

@@ -10,7 +10,6 @@
 
 namespace xj {
 
-constexpr int kDiaSolveRows = 1 << 19;  // smallest operator whose solves run on the diagonals
 
 // Workgroups of the persistent loop: every grid barrier costs each workgroup
 // an L2 write-back and invalidate on its XCD, so a small operator runs best on
@@ -263,10 +262,10 @@ Operator::Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t*
     //
     // The solves' layout follows from the matrix instead, so that operators
     // built from the same matrix give bit-identical iterates whatever the
-    // timings: the diagonals when they hold at most 25% padding slots and the
-    // matrix has at least kDiaSolveRows rows (below that the fp64 kernel's
-    // 512-row workgroups leave the chip half empty: a 512^2 diffusion solve
-    // took 61.9 ms on them against 56.8 ms on the tiles), else
+    // timings: the diagonals when they hold at most 25% padding slots (at
+    // every size: below kDiaWideRows the fp64 kernel takes one row per lane,
+    // so a small operator still fills the chip -- 512^2 diffusion 45.0 ms
+    // against 48.8 ms on the tiles, 27-point 64^3 4.0 against 5.6 ms), else
     // the panels when their rule holds (no int16 column stream, at least
     // kPanelAutoRows rows, columns ascending in panel order), else the tiles
     // or the pattern tiles (whichever SpMV is faster; both give the same
@@ -305,7 +304,7 @@ Operator::Operator(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t*
         }
         auto_solve_ = (t_pat > 0.0f && t_pat < t_tiles) ? kPatterns : kTiles;
         if (panels_ok) auto_solve_ = kPanels;
-        if (has_dia_ && n >= kDiaSolveRows && 4 * (int64_t)A_.dia.ndia * n <= 5 * nnz) auto_solve_ = kDiagonals;
+        if (has_dia_ && 4 * (int64_t)A_.dia.ndia * n <= 5 * nnz) auto_solve_ = kDiagonals;
         const auto used = [&](int L) { return auto_layout_ == L || auto_solve_ == L; };
         if (!used(kPanels)) free_panels();
         if (!used(kPatterns)) free_patterns();

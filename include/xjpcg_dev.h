@@ -85,8 +85,7 @@ XJPCG_Status_t xjpcg_op_set_values(xjpcg_op_t* op, const void* d_val);
 XJPCG_Status_t xjpcg_op_set_layout(xjpcg_op_t* op, int layout, void* stream, int* h_active);
 /* The layout the solves run on.  It follows from the matrix, not from
  * timing, so that operators built from the same matrix give bit-identical
- * iterates: diagonals when they apply with at most 25% padding slots and
- * the operator has at least 2^19 rows, else
+ * iterates: diagonals when they apply with at most 25% padding slots, else
  * the panels when they were built (the rule above), else the tiles or the pattern tiles
  * (the faster SpMV of the two; both give the same iterates).  It can differ
  * from the standalone SpMV's layout.  xjpcg_op_set_layout with layout > 0

@@ -355,8 +355,9 @@ def test_diagonals_value_refresh_and_boundary_rows(cuda):
 
 
 def test_diagonal_solve_odd_rows(cuda):
-    """Diagonal-layout CG on an odd number of rows: the fp64 kernels take two rows per lane, so the last
-    lane's second row is padding (mask 0, sum dropped, no Ap store).  Oracle parity and repeatability."""
+    """Diagonal-layout CG on an odd number of rows below 2^19 (one row per lane; the last workgroup runs
+    partly empty).  Oracle parity and repeatability.  The two-rows-per-lane padding row is covered by the
+    91 x 81 x 81 operator below."""
     from hpc_amd import device
     A = workloads.stencil(workloads.STENCIL_3D7, 21, 13, 7, [0.5, 1.5, 2.5])
     assert A.n % 2 == 1
@@ -375,7 +376,7 @@ def test_diagonal_solve_odd_rows(cuda):
 
 def test_equal_matrices_give_identical_solves(cuda):
     """The solve layout follows from the matrix, not from creation-time timings: two operators built from
-    the same 81^3 27-point matrix (above the diagonal-solve size floor) both solve on the diagonals and give
+    the same 81^3 27-point matrix (two rows per lane from 2^19 rows) both solve on the diagonals and give
     bit-identical iterates, at oracle parity."""
     from hpc_amd import device
     s = workloads.c2_stencil27(81)
@@ -417,7 +418,7 @@ def test_equal_graph_matrices_give_identical_panel_solves(cuda):
 
 
 def test_diagonal_solve_7point_above_floor(cuda):
-    """An anisotropic 3D 7-point operator with an odd number of rows above the diagonal-solve floor: the
+    """An anisotropic 3D 7-point operator with an odd number of rows above 2^19: the
     operator solves on the diagonals by itself (two rows per lane, the last lane's second row padding) and
     matches the oracle."""
     from hpc_amd import device
