@@ -32,13 +32,14 @@ std::mutex g_mu;
 std::map<int, DevicePool> g_pools;  // per device: events belong to the device they were created on
 
 void ready(Slot& sl) {
-    if (!sl.host) {
-        hip_check(hipHostMalloc(&sl.host, kStageChunkBytes, hipHostMallocPortable), "pinned staging chunk");
-        hip_check(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming), "staging event");
-    } else if (sl.pending) {
-        hip_check(hipEventSynchronize(sl.done), "staging chunk reuse");
-    }
+    if (sl.pending) hip_check(hipEventSynchronize(sl.done), "staging chunk reuse");
     sl.pending = false;
+    if (!sl.done) hip_check(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming), "staging event");
+    if (!sl.host) {
+        void* h = nullptr;
+        hip_check(hipHostMalloc(&h, kStageChunkBytes, hipHostMallocPortable), "pinned staging chunk");
+        sl.host = h;
+    }
 }
 
 struct Chunk {
