@@ -74,7 +74,7 @@ def cpu_model():
 
 LAYOUT_NAMES = {1: "tiles", 2: "panels", 3: "patterns", 4: "diagonals"}
 LAYOUT_KERNELS = {1: "spmv_tiles<%s, %s>", 2: "spmv_panels<%s>", 3: "spmv_pattern<%s>", 4: "spmv_dia<%s, %d>"}
-DIA_WIDE_ROWS = 1 << 19   # from this many rows the fp64 diagonal kernel takes two rows per lane
+DIA_WIDE_ROWS = 5 << 16   # from this many rows the fp64 diagonal kernel takes two rows per lane
 TRAFFIC_PROFILES = {("stencil27_160", "tiles"): "spmv_tiles_stencil27_160.md",
                     ("stencil27_160", "patterns"): "spmv_pattern_stencil27_160.md",
                     ("stencil27_160", "diagonals"): "spmv_dia_stencil27_160.md",

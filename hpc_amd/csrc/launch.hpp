@@ -38,7 +38,7 @@ constexpr int kDiaMax = 32;
 // (one 16-byte load per diagonal) from kDiaWideRows rows up, one row for fp32
 // values and for smaller fp64 operators (two rows per lane would leave a
 // small operator's grid at a workgroup or two per CU).
-constexpr int kDiaWideRows = 1 << 19;
+constexpr int kDiaWideRows = 5 << 16;  // 327,680 rows: the measured crossover
 constexpr int dia_rows_per_lane(bool f32, int n) { return f32 || n < kDiaWideRows ? 1 : 2; }
 // Leading dimension of the diagonal value arrays and length of the mask
 // array: n rounded up to even, so a lane's two rows are one aligned 16-byte

@@ -72,7 +72,7 @@ XJPCG_Status_t xjpcg_op_set_values(xjpcg_op_t* op, const void* d_val);
  * column stream, 4 bytes per row instead of 2-4 per entry; same sums as 1);
  * 4 = diagonals (one value array per global offset col - row with zeros where
  * a row has no entry, a 32-bit presence mask per row; each row's entries are
- * summed in column order, two rows per lane for fp64 values from 2^19 rows,
+ * summed in column order, two rows per lane for fp64 values from 327,680 rows,
  * one row per lane otherwise: same sums as 1).
  * At creation the operator times the layouts that apply and keeps the fastest
  * (panels are built for operators of at least 2^18 rows whose columns do not

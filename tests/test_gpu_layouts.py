@@ -355,7 +355,7 @@ def test_diagonals_value_refresh_and_boundary_rows(cuda):
 
 
 def test_diagonal_solve_odd_rows(cuda):
-    """Diagonal-layout CG on an odd number of rows below 2^19 (one row per lane; the last workgroup runs
+    """Diagonal-layout CG on an odd number of rows below 327,680 (one row per lane; the last workgroup runs
     partly empty).  Oracle parity and repeatability.  The two-rows-per-lane padding row is covered by the
     91 x 81 x 81 operator below."""
     from hpc_amd import device
@@ -376,12 +376,12 @@ def test_diagonal_solve_odd_rows(cuda):
 
 def test_equal_matrices_give_identical_solves(cuda):
     """The solve layout follows from the matrix, not from creation-time timings: two operators built from
-    the same 81^3 27-point matrix (two rows per lane from 2^19 rows) both solve on the diagonals and give
+    the same 81^3 27-point matrix (two rows per lane from 327,680 rows) both solve on the diagonals and give
     bit-identical iterates, at oracle parity."""
     from hpc_amd import device
     s = workloads.c2_stencil27(81)
     A = s.A
-    assert A.n >= 1 << 19
+    assert A.n >= 5 << 16
     out = []
     for _ in range(2):
         op = device.CsrOperator(_t(cuda, A.rowptr), _t(cuda, A.col), _t(cuda, A.val), rowptr_host=A.rowptr)
@@ -418,12 +418,12 @@ def test_equal_graph_matrices_give_identical_panel_solves(cuda):
 
 
 def test_diagonal_solve_7point_above_floor(cuda):
-    """An anisotropic 3D 7-point operator with an odd number of rows above 2^19: the
+    """An anisotropic 3D 7-point operator with an odd number of rows above 327,680: the
     operator solves on the diagonals by itself (two rows per lane, the last lane's second row padding) and
     matches the oracle."""
     from hpc_amd import device
     A = workloads.stencil(workloads.STENCIL_3D7, 91, 81, 81, [0.5, 1.5, 2.5])
-    assert A.n % 2 == 1 and A.n >= 1 << 19
+    assert A.n % 2 == 1 and A.n >= 5 << 16
     b = workloads.uniform(6, 1, A.n, -1, 1)
     op = device.CsrOperator(_t(cuda, A.rowptr), _t(cuda, A.col), _t(cuda, A.val), rowptr_host=A.rowptr)
     assert op.solve_layout == 4
