@@ -940,7 +940,8 @@ def main():
         extra = {}
         for name, fn in (("laplace2d_256", bench_laplace2d), ("graph_4m", bench_graph),
                          ("batch_512x16", bench_batched), ("autograd_512", bench_autograd),
-                         ("inexact_fp32_512", bench_inexact_mixed), ("shard_160", bench_shard)):
+                         ("inexact_fp32_512", bench_inexact_mixed), ("shard_160", bench_shard),
+                         ("ingest_coo_160", bench_ingest), ("signature_160", bench_signature)):
             t0 = time.time()
             try:
                 extra[name] = fn(args)
