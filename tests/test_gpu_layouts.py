@@ -555,3 +555,18 @@ def test_solve_layout_chosen_separately(cuda):
     for lay in (1, 3, 4):
         assert op.set_layout(lay) == lay and op.solve_layout == lay
     assert op.set_layout(0) == spmv0 and op.solve_layout == solve0
+
+
+@pytest.mark.parametrize("m", [70, 71])
+def test_diagonal_spmv_two_rows_per_lane_bitexact(cuda, m):
+    """27-point m^3 operators just above 327,680 rows, where the fp64 diagonal kernel takes two rows per lane
+    (71^3 is odd: the last lane's second row is padding): y = A x equals the oracle's left-to-right sums."""
+    from hpc_amd import device
+    A = workloads.stencil27(m)
+    assert A.n >= 5 << 16 and A.n % 2 == m % 2
+    op = device.CsrOperator(_t(cuda, A.rowptr), _t(cuda, A.col), _t(cuda, A.val), rowptr_host=A.rowptr)
+    assert op.set_layout(4) == 4
+    xv = workloads.uniform(12, m, A.n, -1, 1)
+    y = op.spmv(_t(cuda, xv)).cpu().numpy()
+    _say(f"diagonals {m}^3 (n={A.n}): bit-exact check")
+    assert np.array_equal(y, oracle.spmv(A.rowptr, A.col, A.val, xv, threads=8))
