@@ -73,8 +73,9 @@ def cpu_model():
 
 
 LAYOUT_NAMES = {1: "tiles", 2: "panels", 3: "patterns", 4: "diagonals"}
-LAYOUT_KERNELS = {1: "spmv_tiles<%s, %s>", 2: "spmv_panels<%s>", 3: "spmv_pattern<%s>", 4: "spmv_dia<%s, %d>"}
-DIA_WIDE_ROWS = 5 << 16   # from this many rows the fp64 diagonal kernel takes two rows per lane
+LAYOUT_KERNELS = {1: "spmv_tiles<%s, %s>", 2: "spmv_panels<%s>", 3: "spmv_pattern<%s>", 4: "spmv_dia<%s, %d, %d>"}
+DIA_WIDE_ROWS = 5 << 16        # from this many rows the fp64 diagonal kernel takes two rows per lane
+DIA_WIDE_BLOCK_ROWS = 3 << 18  # and from this many, 512-lane workgroups
 TRAFFIC_PROFILES = {("stencil27_160", "tiles"): "spmv_tiles_stencil27_160.md",
                     ("stencil27_160", "patterns"): "spmv_pattern_stencil27_160.md",
                     ("stencil27_160", "diagonals"): "spmv_dia_stencil27_160.md",
@@ -86,7 +87,8 @@ def kernel_name(layout, f32, compact, n):
     if layout == 1:
         return LAYOUT_KERNELS[1] % (vt, "int16_t" if compact else "int")
     if layout == 4:
-        return LAYOUT_KERNELS[4] % (vt, 1 if f32 or n < DIA_WIDE_ROWS else 2)
+        two = not f32 and n >= DIA_WIDE_ROWS
+        return LAYOUT_KERNELS[4] % (vt, 2 if two else 1, 512 if two and n >= DIA_WIDE_BLOCK_ROWS else 256)
     return LAYOUT_KERNELS[layout] % vt
 
 

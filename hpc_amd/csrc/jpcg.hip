@@ -724,7 +724,8 @@ struct DiaLane {
 static_assert(dia_rows_per_lane(true, kDiaWideRows) == 1 && dia_rows_per_lane(false, kDiaWideRows) == 2 &&
                   dia_rows_per_lane(false, kDiaWideRows - 1) == 1,
               "the kernels are instantiated for these rows per lane");
-static_assert(kDiaBlock == kBlock, "diagonal kernels run kBlock lanes");
+static_assert(kDiaBlock == kBlock && kDiaWideBlock % 64 == 0 && kDiaWideBlock <= 1024,
+              "one-row diagonal kernels run kBlock lanes, two-row ones whole wavefronts");
 static_assert(kVectorBlock == kBlock, "vector kernels run kBlock lanes");
 
 template <typename T, int R>
@@ -840,12 +841,12 @@ __device__ __forceinline__ void dia_rows_v(const DiaDev& D, int n, int i0, const
 // forceinline wrapper the compiler gave the fp32 kernels 84 VGPRs instead of
 // 59 (occupancy 5 instead of 8).
 
-// y = A x (or b - A x), R consecutive rows per lane, 256 R rows per workgroup
-template <typename VT, int R>
-__global__ __launch_bounds__(kBlock) void spmv_dia(int n, DiaDev D, const double* __restrict__ dsplit,
-                                                   const double* __restrict__ x, double* __restrict__ y,
-                                                   const double* __restrict__ b) {
-    const int i0 = ((int)blockIdx.x * kBlock + (int)threadIdx.x) * R;
+// y = A x (or b - A x), R consecutive rows per lane, BS R rows per workgroup
+template <typename VT, int R, int BS>
+__global__ __launch_bounds__(BS) void spmv_dia(int n, DiaDev D, const double* __restrict__ dsplit,
+                                               const double* __restrict__ x, double* __restrict__ y,
+                                               const double* __restrict__ b) {
+    const int i0 = ((int)blockIdx.x * BS + (int)threadIdx.x) * R;
     if (i0 >= n) return;
     double s[R];
     if constexpr (R > 1) {
@@ -861,15 +862,15 @@ __global__ __launch_bounds__(kBlock) void spmv_dia(int n, DiaDev D, const double
         if (i0 + r < n) st_stream(y + i0 + r, b ? b[i0 + r] - s[r] : s[r]);
 }
 
-// CG SpMV on the diagonal layout: 256 R consecutive rows per workgroup, R per
+// CG SpMV on the diagonal layout: BS R consecutive rows per workgroup, R per
 // lane (every address follows from the block index, so nothing waits on a
 // table); Ap is stored and each workgroup leaves one p.Ap partial, a lane's
 // rows' products added in row order, an xor-butterfly within each wavefront,
 // then the wavefronts in order -- a fixed order.  With fp32 work vectors (PT =
 // float) p is widened as it is read, Ap is rounded as it is stored and the
 // partial uses the stored value.
-template <typename VT, typename PT, int R>
-__global__ __launch_bounds__(kBlock) void cg_spmv_dia(CgParams P) {
+template <typename VT, typename PT, int R, int BS>
+__global__ __launch_bounds__(BS) void cg_spmv_dia(CgParams P) {
     CgState* st = P.st;
     const int stop = st->stop;
     const int b = (int)blockIdx.x;
@@ -877,7 +878,7 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_dia(CgParams P) {
     const PT* __restrict__ p = wv<PT>(P.p);
     PT* __restrict__ Ap = wv<PT>(P.Ap);
     double pap = 0.0;
-    const int i0 = (b * kBlock + (int)threadIdx.x) * R;
+    const int i0 = (b * BS + (int)threadIdx.x) * R;
     if (i0 < P.n) {
         double pi[R], y[R];
 #pragma unroll
@@ -902,8 +903,8 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_dia(CgParams P) {
     // one partial per workgroup rather than per wavefront: a quarter of the
     // partials for cg_alpha or the vector kernels to fold (160^3 mixed solve
     // with fp32 work vectors 43.7 ms against 44.0 ms, same box)
-    __shared__ double red[kWaves];
-    pap = block_sum(pap, red);
+    __shared__ double red[BS / 64];
+    pap = block_sum_w<BS / 64>(pap, red);
     if (threadIdx.x == 0) P.partA[b] = pap;
     // k advances after the last load: a store ahead of the loads could alias
     // the tables and would turn their scalar loads into vector loads
@@ -1210,20 +1211,26 @@ hipError_t launch_cg_iteration(const CgParams& P, bool f32, bool c16, hipStream_
     } else if (P.dia_on) {
         const int g = dia_groups(P.n, f32);
         const bool wide = dia_rows_per_lane(f32, P.n) == 2;  // fp64 values only
+        constexpr int WB = kDiaWideBlock, NB = kDiaBlock;
+        const bool wb = dia_block(f32, P.n) == WB;
         if (P.w32) {
             if (f32)
-                cg_spmv_dia<float, float, 1><<<g, kBlock, 0, s>>>(P);
+                cg_spmv_dia<float, float, 1, NB><<<g, NB, 0, s>>>(P);
+            else if (wide && wb)
+                cg_spmv_dia<double, float, 2, WB><<<g, WB, 0, s>>>(P);
             else if (wide)
-                cg_spmv_dia<double, float, 2><<<g, kBlock, 0, s>>>(P);
+                cg_spmv_dia<double, float, 2, NB><<<g, NB, 0, s>>>(P);
             else
-                cg_spmv_dia<double, float, 1><<<g, kBlock, 0, s>>>(P);
+                cg_spmv_dia<double, float, 1, NB><<<g, NB, 0, s>>>(P);
         } else {
             if (f32)
-                cg_spmv_dia<float, double, 1><<<g, kBlock, 0, s>>>(P);
+                cg_spmv_dia<float, double, 1, NB><<<g, NB, 0, s>>>(P);
+            else if (wide && wb)
+                cg_spmv_dia<double, double, 2, WB><<<g, WB, 0, s>>>(P);
             else if (wide)
-                cg_spmv_dia<double, double, 2><<<g, kBlock, 0, s>>>(P);
+                cg_spmv_dia<double, double, 2, NB><<<g, NB, 0, s>>>(P);
             else
-                cg_spmv_dia<double, double, 1><<<g, kBlock, 0, s>>>(P);
+                cg_spmv_dia<double, double, 1, NB><<<g, NB, 0, s>>>(P);
         }
     } else if (P.pat_on) {
         if (f32)
@@ -1262,11 +1269,13 @@ hipError_t launch_spmv(const CsrDev& A, const double* x, double* y, const double
         const int g = dia_groups(A.n, A.val_f32);
         if (g <= 0) return hipSuccess;
         if (A.val_f32)
-            spmv_dia<float, 1><<<g, kBlock, 0, s>>>(A.n, A.dia, A.dsplit, x, y, b);
+            spmv_dia<float, 1, kDiaBlock><<<g, kDiaBlock, 0, s>>>(A.n, A.dia, A.dsplit, x, y, b);
+        else if (dia_rows_per_lane(false, A.n) == 2 && dia_block(false, A.n) == kDiaWideBlock)
+            spmv_dia<double, 2, kDiaWideBlock><<<g, kDiaWideBlock, 0, s>>>(A.n, A.dia, A.dsplit, x, y, b);
         else if (dia_rows_per_lane(false, A.n) == 2)
-            spmv_dia<double, 2><<<g, kBlock, 0, s>>>(A.n, A.dia, A.dsplit, x, y, b);
+            spmv_dia<double, 2, kDiaBlock><<<g, kDiaBlock, 0, s>>>(A.n, A.dia, A.dsplit, x, y, b);
         else
-            spmv_dia<double, 1><<<g, kBlock, 0, s>>>(A.n, A.dia, A.dsplit, x, y, b);
+            spmv_dia<double, 1, kDiaBlock><<<g, kDiaBlock, 0, s>>>(A.n, A.dia, A.dsplit, x, y, b);
         return hipGetLastError();
     }
     if (A.ntiles <= 0) return hipSuccess;

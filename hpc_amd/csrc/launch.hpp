@@ -46,9 +46,18 @@ constexpr int dia_rows_per_lane(bool f32, int n) { return f32 || n < kDiaWideRow
 constexpr int dia_ld(int n) { return (n + 1) & ~1; }
 // workgroups of the diagonal kernels (kDiaBlock lanes each) and the CG
 // SpMV's p.Ap partials (one per workgroup)
+// (kDiaWideBlock lanes from kDiaWideBlockRows rows with two rows per lane:
+// 1,024 rows per workgroup against 512, 27-point SpMV 100^3 48.5 / 51.5 us,
+// 128^3 89.5 / 95.0, 160^3 168 / 172.5; 81^3 40.3 / 31.5, too few workgroups)
 constexpr int kDiaBlock = 256;
+constexpr int kDiaWideBlock = 512;
+constexpr int kDiaWideBlockRows = 3 << 18;  // 786,432
+constexpr int dia_block(bool f32, int n) {
+    return dia_rows_per_lane(f32, n) == 2 && n >= kDiaWideBlockRows ? kDiaWideBlock : kDiaBlock;
+}
 constexpr int dia_groups(int n, bool f32) {
-    return (n + kDiaBlock * dia_rows_per_lane(f32, n) - 1) / (kDiaBlock * dia_rows_per_lane(f32, n));
+    const int rows = dia_block(f32, n) * dia_rows_per_lane(f32, n);
+    return (n + rows - 1) / rows;
 }
 constexpr int dia_cg_partials(int n, bool f32) { return dia_groups(n, f32); }
 // Workgroups of the CG vector kernels (256 lanes, grid-stride; one r.r / r.z
