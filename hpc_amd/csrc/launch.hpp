@@ -48,7 +48,9 @@ constexpr int dia_ld(int n) { return (n + 1) & ~1; }
 // SpMV's p.Ap partials (one per workgroup)
 // (kDiaWideBlock lanes from kDiaWideBlockRows rows with two rows per lane:
 // 1,024 rows per workgroup against 512, 27-point SpMV 100^3 48.5 / 51.5 us,
-// 128^3 89.5 / 95.0, 160^3 168 / 172.5; 81^3 40.3 / 31.5, too few workgroups)
+// 128^3 89.5 / 95.0, 160^3 168 / 172.5; 81^3 40.3 / 31.5, too few workgroups.
+// One row per lane keeps 256: the 160^3 fp32-value SpMV took 119.6-120.6 us
+// with 256 or 128 lanes, 121.5-121.8 with 512, 122.5-123.2 with 1024)
 constexpr int kDiaBlock = 256;
 constexpr int kDiaWideBlock = 512;
 constexpr int kDiaWideBlockRows = 3 << 18;  // 786,432
