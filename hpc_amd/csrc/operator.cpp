@@ -397,10 +397,15 @@ bool Operator::build_diagonals(hipStream_t s) {
         hip_check(hipMemcpyAsync(&fail, abort_.ptr, sizeof(int), hipMemcpyDeviceToHost, s), "tile offsets");
         hip_check(hipStreamSynchronize(s), "tile offsets");
         if (fail) return false;
-        for (int32_t v : h)
-            if (v != INT32_MIN) offs.push_back(v);
-        std::sort(offs.begin(), offs.end());
-        offs.erase(std::unique(offs.begin(), offs.end()), offs.end());
+        // distinct offsets, ascending, kept in a short sorted list: sorting
+        // every tile's copy (1.45M values on the 160^3 stencil) took 21 ms
+        for (int32_t v : h) {
+            if (v == INT32_MIN) continue;
+            const auto it = std::lower_bound(offs.begin(), offs.end(), v);
+            if (it != offs.end() && *it == v) continue;
+            if ((int)offs.size() > kDiaMax) break;  // already more than the layout holds
+            offs.insert(it, v);
+        }
     }
     const int ndia = (int)offs.size();
     if (ndia == 0 || ndia > kDiaMax || (int64_t)ndia * A_.n > 2 * A_.nnz) return false;
