@@ -183,3 +183,40 @@ def test_pcg_coo_solver_staged_uploads_match_oracle(cuda):
     assert out.status == Status.SUCCESS
     assert abs(out.iterations - ref["iter"]) <= 1
     assert np.linalg.norm(out.x - ref["x"]) / np.linalg.norm(ref["x"]) <= 1e-10
+
+
+def test_two_handles_upload_concurrently(cuda):
+    """Two pcg.h handles used from two host threads at once, each with its own shuffled COO large enough for
+    the pinned staging pool (which serialises uploads on one lock): both solves match the oracle."""
+    import threading
+    from hpc_amd.pcg import JPCG, Status
+    cases = []
+    for m, seed in ((300, 21), (340, 22)):
+        s = workloads.c1_laplace2d(m)
+        rows, cols, vals = _shuffled_coo(s.A, seed)
+        assert rows.size * 16 > 4 << 20
+        rp, col, val, _ = oracle.coo_to_csr(s.A.n, rows, cols, vals)
+        ref = oracle.jpcg(rp, col, val, s.A.diag, s.b, 3000, 1e-10, threads=4)
+        cases.append((s, rows, cols, vals, ref))
+    out = [None, None]
+    errors = []
+
+    def run(i):
+        try:
+            s, rows, cols, vals, _ = cases[i]
+            with JPCG(0) as h:
+                out[i] = h.coo_solve(s.A.n, rows, cols, vals, s.A.diag, s.b, 3000, 1e-10)
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    th = [threading.Thread(target=run, args=(i,)) for i in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=120)
+    assert not errors, errors
+    for (s, _, _, _, ref), res in zip(cases, out):
+        _say(f"n={s.A.n}: {res.iterations} iterations (oracle {ref['iter']}), status {res.status}")
+        assert res.status == Status.SUCCESS
+        assert abs(res.iterations - ref["iter"]) <= 1
+        assert np.linalg.norm(res.x - ref["x"]) / np.linalg.norm(ref["x"]) <= 1e-10
