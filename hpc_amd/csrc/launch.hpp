@@ -50,7 +50,9 @@ constexpr int dia_ld(int n) { return (n + 1) & ~1; }
 // 1,024 rows per workgroup against 512, 27-point SpMV 100^3 48.5 / 51.5 us,
 // 128^3 89.5 / 95.0, 160^3 168 / 172.5; 81^3 40.3 / 31.5, too few workgroups.
 // One row per lane keeps 256: the 160^3 fp32-value SpMV took 119.6-120.6 us
-// with 256 or 128 lanes, 121.5-121.8 with 512, 122.5-123.2 with 1024)
+// with 256 or 128 lanes, 121.5-121.8 with 512, 122.5-123.2 with 1024; 128
+// lanes below 2^18 rows slowed the 256^2 and 400^2 solves by 6-10%, twice
+// the p.Ap partials per iteration)
 constexpr int kDiaBlock = 256;
 constexpr int kDiaWideBlock = 512;
 constexpr int kDiaWideBlockRows = 3 << 18;  // 786,432
